@@ -1,0 +1,372 @@
+"""Python mirror of the Device interface of include/astaroth.h.
+
+Loads astaroth_amd/lib/libastaroth_{fp32,fp64}.so with ctypes and wraps the C entry points with
+the reference's names and argument meaning (src/core/device.cc, include/astaroth.h:501-642), so a
+test reads like the reference's own samples:
+
+    lib  = Library(double=True)
+    info = lib.config(n=64)                    # AcMeshInfo of config 1
+    mesh = HostMesh(lib, info)                 # 8 numpy arrays of m^3 cells
+    dev  = lib.device_create(0, info)          # acDeviceCreate
+    dev.load_mesh(mesh); dev.integrate_step(dt); dev.store_mesh(mesh)
+
+The library is required: a missing or unloadable .so raises, there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import IP, RP, VTX, RT, int3
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(PKG, "lib")
+
+
+class AstarothError(RuntimeError):
+    pass
+
+
+def lib_path(double: bool) -> str:
+    return os.path.join(LIBDIR, "libastaroth_%s.so" % ("fp64" if double else "fp32"))
+
+
+_LIBS: dict[bool, "Library"] = {}
+
+
+class Library:
+    """One precision build of the C library."""
+
+    def __new__(cls, double: bool = False):
+        if double in _LIBS:
+            return _LIBS[double]
+        self = super().__new__(cls)
+        _LIBS[double] = self
+        self._init(double)
+        return self
+
+    def _init(self, double: bool):
+        path = lib_path(double)
+        if not os.path.exists(path):
+            raise AstarothError(
+                "%s is not built; run `python -m astaroth_amd.build` (hipcc, gfx950)" % path)
+        self.path = path
+        self.double = double
+        self.dll = ctypes.CDLL(path)
+        self.real = abi.real_ctype(double)
+        self.dtype = abi.real_dtype(double)
+        self.MeshInfo, self.Mesh = abi.structs(double)
+        self._declare()
+        size_mi = ctypes.c_size_t()
+        size_m = ctypes.c_size_t()
+        rsize = self.dll.acAbiRealSize(ctypes.byref(size_mi), ctypes.byref(size_m))
+        if rsize != ctypes.sizeof(self.real) or size_mi.value != ctypes.sizeof(self.MeshInfo) \
+                or size_m.value != ctypes.sizeof(self.Mesh):
+            raise AstarothError("ABI mismatch between %s and astaroth_amd.abi" % path)
+
+    # ---- ctypes declarations -------------------------------------------------------------
+    def _declare(self):
+        d = self.dll
+        R = self.real
+        MI = self.MeshInfo
+        M = self.Mesh
+        vp = ctypes.c_void_p
+        i = ctypes.c_int
+        I3 = abi.Int3
+        decl = {
+            "acAbiRealSize": (i, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+            "acDeviceCreate": (i, [i, MI, ctypes.POINTER(vp)]),
+            "acDeviceDestroy": (i, [vp]),
+            "acDevicePrintInfo": (i, [vp]),
+            "acDeviceAutoOptimize": (i, [vp]),
+            "acDeviceSynchronizeStream": (i, [vp, i]),
+            "acDeviceSwapBuffers": (i, [vp]),
+            "acDeviceLoadScalarUniform": (i, [vp, i, i, R]),
+            "acDeviceLoadIntUniform": (i, [vp, i, i, i]),
+            "acDeviceLoadInt3Uniform": (i, [vp, i, i, I3]),
+            "acDeviceLoadMeshInfo": (i, [vp, MI]),
+            "acDeviceLoadDefaultUniforms": (i, [vp]),
+            "acDeviceLoadVertexBufferWithOffset": (i, [vp, i, M, i, I3, I3, i]),
+            "acDeviceLoadMeshWithOffset": (i, [vp, i, M, I3, I3, i]),
+            "acDeviceLoadVertexBuffer": (i, [vp, i, M, i]),
+            "acDeviceLoadMesh": (i, [vp, i, M]),
+            "acDeviceSetVertexBuffer": (i, [vp, i, i, R]),
+            "acDeviceStoreVertexBufferWithOffset": (i, [vp, i, i, I3, I3, i, ctypes.POINTER(M)]),
+            "acDeviceStoreMeshWithOffset": (i, [vp, i, I3, I3, i, ctypes.POINTER(M)]),
+            "acDeviceStoreVertexBuffer": (i, [vp, i, i, ctypes.POINTER(M)]),
+            "acDeviceStoreMesh": (i, [vp, i, ctypes.POINTER(M)]),
+            "acDeviceTransferVertexBuffer": (i, [vp, i, i, vp]),
+            "acDeviceTransferMesh": (i, [vp, i, vp]),
+            "acDeviceIntegrateSubstep": (i, [vp, i, i, I3, I3, R]),
+            "acDevicePeriodicBoundcondStep": (i, [vp, i, i, I3, I3]),
+            "acDevicePeriodicBoundconds": (i, [vp, i, I3, I3]),
+            "acDeviceGeneralBoundconds": (i, [vp, i, I3, I3, MI, I3]),
+            "acDeviceReduceScal": (i, [vp, i, i, i, ctypes.POINTER(R)]),
+            "acDeviceReduceVec": (i, [vp, i, i, i, i, i, ctypes.POINTER(R)]),
+            "acDeviceReduceVecScal": (i, [vp, i, i, i, i, i, i, ctypes.POINTER(R)]),
+            "acDeviceReduceScalDouble": (i, [vp, i, i, i, ctypes.POINTER(ctypes.c_double)]),
+            "acDeviceReduceVecDouble": (i, [vp, i, i, i, i, i, ctypes.POINTER(ctypes.c_double)]),
+            "acDeviceReduceVecScalDouble": (i, [vp, i, i, i, i, i, i, ctypes.POINTER(ctypes.c_double)]),
+            "acDeviceRunMPITest": (i, []),
+            "acDeviceIntegrateStep": (i, [vp, i, R]),
+            "acDeviceIntegrateStepGraph": (i, [vp, i, R]),
+            "acKernelConfigCount": (i, []),
+            "acKernelConfigGet": (i, [i, ctypes.POINTER(abi.KernelConfig)]),
+            "acDeviceGetKernelConfig": (i, [vp, ctypes.POINTER(abi.KernelConfig)]),
+            "acDeviceSetKernelConfig": (i, [vp, abi.KernelConfig]),
+            "acDeviceTimeKernelConfig": (ctypes.c_float, [vp, abi.KernelConfig, i]),
+            "acDeviceRandomizeUniform": (i, [vp, i, i, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong]),
+            "acDeviceGetStreamHandle": (vp, [vp, i]),
+            "acHostUpdateBuiltinParams": (i, [ctypes.POINTER(MI)]),
+            "acHostMeshCreate": (i, [MI, ctypes.POINTER(M)]),
+            "acHostMeshRandomize": (i, [ctypes.POINTER(M)]),
+            "acHostMeshDestroy": (i, [ctypes.POINTER(M)]),
+            "acLoadConfig": (i, [ctypes.c_char_p, ctypes.POINTER(MI)]),
+        }
+        self.symbols = sorted(decl)
+        for name, (res, args) in decl.items():
+            fn = getattr(d, name)
+            fn.restype = res
+            fn.argtypes = args
+
+    # ---- configuration helpers -----------------------------------------------------------
+    def update_builtin_params(self, info):
+        self.dll.acHostUpdateBuiltinParams(ctypes.byref(info))
+        return info
+
+    def load_config(self, path: str):
+        info = self.MeshInfo()
+        res = self.dll.acLoadConfig(path.encode(), ctypes.byref(info))
+        return info, res
+
+    def config(self, n=64, ny=None, nz=None, **overrides):
+        """AcMeshInfo of the 8-field MHD configuration used by the tests and the benchmark.
+
+        Grid spacing 2 pi / n, nu = eta = 5e-3, cs = 1, gamma = 5/3, zeta = 0.01, mu0 = 1.4,
+        cp = 1, lnT0 = 1.2, lnrho0 = 1.3 (remaining values from the reference's
+        config/astaroth.conf:42-67). `overrides` set any real parameter by name.
+        """
+        info = self.MeshInfo()
+        ctypes.memset(ctypes.byref(info), 0, ctypes.sizeof(info))
+        nx = int(n)
+        ny = int(ny if ny is not None else n)
+        nz = int(nz if nz is not None else n)
+        info.int_params[IP["AC_nx"]] = nx
+        info.int_params[IP["AC_ny"]] = ny
+        info.int_params[IP["AC_nz"]] = nz
+        vals = dict(
+            AC_dsx=2 * math.pi / nx, AC_dsy=2 * math.pi / ny, AC_dsz=2 * math.pi / nz,
+            AC_nu_visc=5e-3, AC_eta=5e-3, AC_cs_sound=1.0, AC_gamma=5.0 / 3.0, AC_zeta=0.01,
+            AC_mu0=1.4, AC_cp_sound=1.0, AC_lnT0=1.2, AC_lnrho0=1.3, AC_cdt=0.4, AC_cdtv=0.3,
+            AC_cdts=1.0,
+        )
+        vals.update(overrides)
+        for k, v in vals.items():
+            info.real_params[RP[k]] = v
+        info.real_params[RP["AC_cs2_sound"]] = info.real_params[RP["AC_cs_sound"]] ** 2
+        self.update_builtin_params(info)
+        return info
+
+    def device_create(self, device_id: int, info, autotune: bool | None = None) -> "Device":
+        return Device(self, device_id, info, autotune)
+
+    def kernel_configs(self):
+        out = []
+        for k in range(self.dll.acKernelConfigCount()):
+            c = abi.KernelConfig()
+            self.dll.acKernelConfigGet(k, ctypes.byref(c))
+            out.append(c)
+        return out
+
+
+def dims_of(info):
+    return tuple(int(info.int_params[IP[k]]) for k in ("AC_nx", "AC_ny", "AC_nz"))
+
+
+def real_params_double(info) -> np.ndarray:
+    return np.array([float(info.real_params[k]) for k in range(abi.NUM_REAL_PARAMS)], dtype=np.float64)
+
+
+class HostMesh:
+    """AcMesh whose eight buffers are numpy arrays of shape (mz, my, mx) (x fastest)."""
+
+    def __init__(self, lib: Library, info, fields=None):
+        self.lib = lib
+        self.info = info
+        nx, ny, nz = dims_of(info)
+        self.shape = (nz + 6, ny + 6, nx + 6)
+        if fields is None:
+            self.fields = [np.zeros(self.shape, dtype=lib.dtype) for _ in range(abi.NUM_VTXBUF_HANDLES)]
+        else:
+            self.fields = [np.ascontiguousarray(f, dtype=lib.dtype) for f in fields]
+            for f in self.fields:
+                if f.shape != self.shape:
+                    raise ValueError("field shape %s != %s" % (f.shape, self.shape))
+        self.struct = lib.Mesh()
+        self.struct.info = info
+        self._sync_pointers()
+
+    def _sync_pointers(self):
+        P = ctypes.POINTER(self.lib.real)
+        for q, f in enumerate(self.fields):
+            self.struct.vertex_buffer[q] = f.ctypes.data_as(P)
+
+    def copy(self) -> "HostMesh":
+        return HostMesh(self.lib, self.info, [f.copy() for f in self.fields])
+
+    def interior(self, q):
+        return self.fields[q][3:-3, 3:-3, 3:-3]
+
+    def __getitem__(self, q):
+        return self.fields[q]
+
+
+class Device:
+    """Handle returned by acDeviceCreate (include/astaroth.h:507)."""
+
+    def __init__(self, lib: Library, device_id: int, info, autotune=None):
+        self.lib = lib
+        self.info = info
+        self.dims = dims_of(info)
+        self.handle = ctypes.c_void_p()
+        old = os.environ.get("AC_AUTOTUNE")
+        if autotune is not None:
+            os.environ["AC_AUTOTUNE"] = "1" if autotune else "0"
+        try:
+            res = lib.dll.acDeviceCreate(int(device_id), info, ctypes.byref(self.handle))
+        finally:
+            if autotune is not None:
+                if old is None:
+                    del os.environ["AC_AUTOTUNE"]
+                else:
+                    os.environ["AC_AUTOTUNE"] = old
+        if res != abi.AC_SUCCESS:
+            raise AstarothError("acDeviceCreate failed")
+
+    # context manager
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+    def _check(self, res, what):
+        if res != abi.AC_SUCCESS:
+            raise AstarothError("%s returned AC_FAILURE" % what)
+        return res
+
+    def destroy(self):
+        if self.handle:
+            self.lib.dll.acDeviceDestroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def synchronize(self, stream=abi.STREAM_ALL):
+        return self._check(self.lib.dll.acDeviceSynchronizeStream(self.handle, stream), "acDeviceSynchronizeStream")
+
+    def swap_buffers(self):
+        return self.lib.dll.acDeviceSwapBuffers(self.handle)
+
+    def load_mesh(self, mesh: HostMesh, stream=abi.STREAM_DEFAULT):
+        self._check(self.lib.dll.acDeviceLoadMesh(self.handle, stream, mesh.struct), "acDeviceLoadMesh")
+        self.synchronize(stream)
+
+    def store_mesh(self, mesh: HostMesh, stream=abi.STREAM_DEFAULT):
+        self._check(self.lib.dll.acDeviceStoreMesh(self.handle, stream, ctypes.byref(mesh.struct)), "acDeviceStoreMesh")
+        self.synchronize(stream)
+        return mesh
+
+    def load_uniform(self, name: str, value, stream=abi.STREAM_DEFAULT):
+        return self.lib.dll.acDeviceLoadScalarUniform(self.handle, stream, RP[name], value)
+
+    def integrate_substep(self, step, start=None, end=None, dt=1e-3, stream=abi.STREAM_DEFAULT):
+        nx, ny, nz = self.dims
+        start = start or (3, 3, 3)
+        end = end or (nx + 3, ny + 3, nz + 3)
+        return self.lib.dll.acDeviceIntegrateSubstep(self.handle, stream, step, int3(*start), int3(*end), dt)
+
+    def periodic_boundconds(self, start=None, end=None, stream=abi.STREAM_DEFAULT):
+        nx, ny, nz = self.dims
+        start = start or (0, 0, 0)
+        end = end or (nx + 6, ny + 6, nz + 6)
+        return self._check(self.lib.dll.acDevicePeriodicBoundconds(self.handle, stream, int3(*start), int3(*end)),
+                           "acDevicePeriodicBoundconds")
+
+    def periodic_boundcond_step(self, handle, start=None, end=None, stream=abi.STREAM_DEFAULT):
+        nx, ny, nz = self.dims
+        start = start or (0, 0, 0)
+        end = end or (nx + 6, ny + 6, nz + 6)
+        return self.lib.dll.acDevicePeriodicBoundcondStep(self.handle, stream, handle, int3(*start), int3(*end))
+
+    def integrate_step(self, dt, stream=abi.STREAM_DEFAULT, graph=False):
+        fn = self.lib.dll.acDeviceIntegrateStepGraph if graph else self.lib.dll.acDeviceIntegrateStep
+        return self._check(fn(self.handle, stream, dt), "acDeviceIntegrateStep")
+
+    def reduce_scal(self, rtype, handle, stream=abi.STREAM_DEFAULT, double=False):
+        if double:
+            out = ctypes.c_double()
+            res = self.lib.dll.acDeviceReduceScalDouble(self.handle, stream, _rt(rtype), _vh(handle), ctypes.byref(out))
+        else:
+            out = self.lib.real()
+            res = self.lib.dll.acDeviceReduceScal(self.handle, stream, _rt(rtype), _vh(handle), ctypes.byref(out))
+        self._check(res, "acDeviceReduceScal")
+        return out.value
+
+    def reduce_vec(self, rtype, h0, h1, h2, stream=abi.STREAM_DEFAULT, double=False):
+        args = (self.handle, stream, _rt(rtype), _vh(h0), _vh(h1), _vh(h2))
+        if double:
+            out = ctypes.c_double()
+            res = self.lib.dll.acDeviceReduceVecDouble(*args, ctypes.byref(out))
+        else:
+            out = self.lib.real()
+            res = self.lib.dll.acDeviceReduceVec(*args, ctypes.byref(out))
+        self._check(res, "acDeviceReduceVec")
+        return out.value
+
+    def reduce_vec_scal(self, rtype, h0, h1, h2, h3, stream=abi.STREAM_DEFAULT, double=False):
+        args = (self.handle, stream, _rt(rtype), _vh(h0), _vh(h1), _vh(h2), _vh(h3))
+        if double:
+            out = ctypes.c_double()
+            res = self.lib.dll.acDeviceReduceVecScalDouble(*args, ctypes.byref(out))
+        else:
+            out = self.lib.real()
+            res = self.lib.dll.acDeviceReduceVecScal(*args, ctypes.byref(out))
+        self._check(res, "acDeviceReduceVecScal")
+        return out.value
+
+    def set_vertex_buffer(self, handle, value, stream=abi.STREAM_DEFAULT):
+        return self._check(self.lib.dll.acDeviceSetVertexBuffer(self.handle, stream, _vh(handle), value),
+                           "acDeviceSetVertexBuffer")
+
+    def randomize_uniform(self, handle, lo, hi, seed, stream=abi.STREAM_DEFAULT):
+        return self._check(self.lib.dll.acDeviceRandomizeUniform(self.handle, stream, _vh(handle), lo, hi, seed),
+                           "acDeviceRandomizeUniform")
+
+    def kernel_config(self) -> abi.KernelConfig:
+        c = abi.KernelConfig()
+        self.lib.dll.acDeviceGetKernelConfig(self.handle, ctypes.byref(c))
+        return c
+
+    def set_kernel_config(self, variant, tile_x, tile_y, z_depth=0):
+        c = abi.KernelConfig(variant, tile_x, tile_y, z_depth)
+        return self._check(self.lib.dll.acDeviceSetKernelConfig(self.handle, c), "acDeviceSetKernelConfig")
+
+    def time_kernel_config(self, config: abi.KernelConfig, iters=5) -> float:
+        return float(self.lib.dll.acDeviceTimeKernelConfig(self.handle, config, iters))
+
+    def autotune(self):
+        return self._check(self.lib.dll.acDeviceAutoOptimize(self.handle), "acDeviceAutoOptimize")
+
+    def stream_handle(self, stream=abi.STREAM_DEFAULT):
+        return self.lib.dll.acDeviceGetStreamHandle(self.handle, stream)
+
+
+def _rt(r):
+    return RT[r] if isinstance(r, str) else int(r)
+
+
+def _vh(h):
+    return VTX[h] if isinstance(h, str) else int(h)

@@ -1,0 +1,122 @@
+// Internal declarations shared by the translation units of libastaroth_{fp32,fp64}.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <utility>
+
+#include "astaroth.h"
+#include "astaroth_utils.h"
+
+#define AC_HIP_CHECK(call)                                                                         \
+    do {                                                                                           \
+        hipError_t err_ = (call);                                                                  \
+        if (err_ != hipSuccess) {                                                                  \
+            fprintf(stderr, "HIP error %s (%d) at %s:%d: %s\n", hipGetErrorString(err_),          \
+                    (int)err_, __FILE__, __LINE__, #call);                                         \
+            abort();                                                                               \
+        }                                                                                          \
+    } while (0)
+
+#define AC_FATAL(...)                                                                              \
+    do {                                                                                           \
+        fprintf(stderr, "astaroth fatal error at %s:%d: ", __FILE__, __LINE__);                    \
+        fprintf(stderr, __VA_ARGS__);                                                              \
+        fprintf(stderr, "\n");                                                                     \
+        abort();                                                                                   \
+    } while (0)
+
+namespace ac {
+
+constexpr int NF = NUM_VTXBUF_HANDLES; // 8 evolved fields
+
+struct VertexBufferArray {
+    AcReal* in[NF];
+    AcReal* out[NF];
+};
+
+// Everything the solve kernel reads besides the buffers: passed by value at every launch, so
+// a captured graph and concurrent Devices each carry their own copy.
+struct PhysParams {
+    // FD coefficients with the grid spacing folded in:
+    //   first  : d1[a][i] = a_i / ds_a
+    //   second : d2[a][i] = b_i / ds_a^2
+    //   cross  : cxy[i] = c_i / (dsx dsy), cxz, cyz likewise
+    AcReal d1[3][4];
+    AcReal d2[3][4];
+    AcReal d2c; // b_0 (1/dsx^2 + 1/dsy^2 + 1/dsz^2): centre weight of a Laplacian
+    AcReal cxy[4], cxz[4], cyz[4];
+    // physics
+    AcReal nu_visc, eta, inv_mu0, eta_mu0, cp_sound, inv_cp, gamma, gamma_m1, gamma_inv_cp;
+    AcReal lnT0, exp_lnT0, lnrho0, zeta, cs2_sound, k_thermal;
+};
+
+// RK3 substep coefficients (integration.cuh:14-37 form): for s = 0
+//   out = in + beta_dt * F
+// for s > 0
+//   out = in + beta_s * (ab * (in - prev) + dt * F),  ab = alpha_s / beta_{s-1}
+struct RK3Coeffs {
+    int step;
+    AcReal dt;
+    AcReal beta;
+    AcReal ab;
+    AcReal beta_dt;
+};
+
+struct SubstepArgs {
+    const AcReal* __restrict__ in[NF];
+    AcReal* __restrict__ out[NF];
+    int mx, my, mxy;
+    int3 start, end;
+    PhysParams phys;
+    RK3Coeffs rk;
+};
+
+PhysParams make_phys_params(const AcMeshInfo& info);
+RK3Coeffs make_rk3_coeffs(int step, AcReal dt);
+
+// Solve-kernel launch (solve.hip). Returns hipSuccess or the launch error; `config` must be
+// one of the compiled configurations.
+hipError_t launch_substep(const AcKernelConfig& config, const SubstepArgs& args, hipStream_t stream);
+bool kernel_config_supported(const AcKernelConfig& config);
+int kernel_config_count();
+AcKernelConfig kernel_config_at(int index);
+AcKernelConfig default_kernel_config(int nx, int ny, int nz);
+
+// Periodic fill of the ghost cells inside [start, end) of `nfields` buffers (boundconds.hip).
+hipError_t launch_periodic_fill(AcReal* const* fields, int nfields, int nx, int ny, int nz,
+                                int3 start, int3 end, hipStream_t stream);
+
+// Reductions (reductions.hip). `vals` are 1, 3 or 4 field pointers. Writes the FP64 result to
+// *d_result (device) and returns the launch status.
+hipError_t launch_reduce(int rtype, int nvals, const AcReal* const* vals, const AcMeshInfo& info,
+                         double* d_partials, int max_partials, double* d_result,
+                         hipStream_t stream);
+int reduce_partials_needed();
+
+// Device-side PRNG fill (init.hip): value = lo + (hi - lo) * u, u = splitmix64(seed, cell)
+// as a double in [0, 1), over the interior [n_min, n_max) of one buffer.
+hipError_t launch_fill_uniform(AcReal* field, const AcMeshInfo& info, double lo, double hi,
+                               unsigned long long seed, hipStream_t stream);
+hipError_t launch_fill_constant(AcReal* field, long n, AcReal value, hipStream_t stream);
+
+} // namespace ac
+
+struct device_s {
+    int id;
+    AcMeshInfo local_config; // creation-time configuration: grid sizes and allocation
+    AcMeshInfo uniforms;     // parameters the kernels use (the reference's d_mesh_info)
+    hipStream_t streams[NUM_STREAMS];
+    ac::VertexBufferArray vba;
+    double* reduce_partials;
+    double* reduce_result;
+    double* reduce_result_host; // pinned
+    AcKernelConfig kernel_config;
+    // acDeviceIntegrateStepGraph cache: (stream, dt bits) -> (graph exec, which in/out parity
+    // the graph was recorded for)
+    std::map<std::pair<int, unsigned long long>, hipGraphExec_t> graphs[2];
+    int parity; // number of swaps mod 2 since creation
+    AcReal* base_in[ac::NF];
+};

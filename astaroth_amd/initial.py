@@ -1,0 +1,80 @@
+"""Synthetic initial conditions of the benchmark configurations.
+
+The uniform generator is the counter-based splitmix64 stream that the library's
+acDeviceRandomizeUniform evaluates on the GPU (csrc/init.hip); `uniform_interior` reproduces it
+bit for bit on the host so small cases can be generated on either side. Seeds are derived from
+(config id, box id, field id) and are recorded with every result.
+
+Config 1-3 distributions: lnrho ~ U(-1e-2, 1e-2); u and A components ~ U(-1e-3, 1e-3);
+entropy = 0 (config 3 adds u_x += 1e-2 sin(2 z)).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .abi import VTX
+
+GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+M1 = np.uint64(0xBF58476D1CE4E5B9)
+M2 = np.uint64(0x94D049BB133111EB)
+
+FIELD_RANGES = {
+    "VTXBUF_LNRHO": (-1e-2, 1e-2),
+    "VTXBUF_UUX": (-1e-3, 1e-3),
+    "VTXBUF_UUY": (-1e-3, 1e-3),
+    "VTXBUF_UUZ": (-1e-3, 1e-3),
+    "VTXBUF_AX": (-1e-3, 1e-3),
+    "VTXBUF_AY": (-1e-3, 1e-3),
+    "VTXBUF_AZ": (-1e-3, 1e-3),
+    "VTXBUF_ENTROPY": (0.0, 0.0),
+}
+
+
+def seed_for(config_id: int, box_id: int, field_id: int) -> int:
+    return (config_id * 1000003 + box_id * 1009 + field_id) & 0xFFFFFFFFFFFFFFFF
+
+
+def uniform_interior(nx, ny, nz, lo, hi, seed) -> np.ndarray:
+    """lo + (hi - lo) u over the interior, shape (nz, ny, nx), FP64."""
+    with np.errstate(over="ignore"):
+        t = np.arange(1, nx * ny * nz + 1, dtype=np.uint64)
+        z = np.uint64(seed) + t * GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * M1
+        z = (z ^ (z >> np.uint64(27))) * M2
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return (lo + (hi - lo) * u).reshape(nz, ny, nx)
+
+
+def periodic_fill(f: np.ndarray) -> np.ndarray:
+    """Ghost cells of one (mz, my, mx) array from the wrapped interior (in place)."""
+    f[...] = np.pad(f[3:-3, 3:-3, 3:-3], 3, mode="wrap")
+    return f
+
+
+def mhd_fields(nx, ny, nz, dtype=np.float64, config_id=1, box_id=0, perturb_sin2z=False):
+    """Eight (mz, my, mx) arrays of the config-1 distribution, ghosts filled periodically."""
+    out = []
+    for name, (lo, hi) in FIELD_RANGES.items():
+        q = VTX[name]
+        f = np.zeros((nz + 6, ny + 6, nx + 6), dtype=np.float64)
+        if hi != lo:
+            f[3:-3, 3:-3, 3:-3] = uniform_interior(nx, ny, nz, lo, hi, seed_for(config_id, box_id, q))
+        if perturb_sin2z and name == "VTXBUF_UUX":
+            dsz = 2 * np.pi / nz
+            zc = (np.arange(nz) * dsz)[:, None, None]
+            f[3:-3, 3:-3, 3:-3] += 1e-2 * np.sin(2 * zc)
+        out.append(periodic_fill(f).astype(dtype))
+    return out
+
+
+def device_init_mhd(dev, config_id=1, box_id=0):
+    """The same distribution generated on the GPU (acDeviceRandomizeUniform + periodic fill)."""
+    for name, (lo, hi) in FIELD_RANGES.items():
+        q = VTX[name]
+        if hi == lo:
+            dev.set_vertex_buffer(q, lo)
+        else:
+            dev.randomize_uniform(q, lo, hi, seed_for(config_id, box_id, q))
+    dev.periodic_boundconds()
+    dev.synchronize()

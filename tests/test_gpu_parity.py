@@ -1,0 +1,344 @@
+"""GPU parity of the HIP library against the oracle (CPU restatement of the reference host model
+solver), through the C ABI.
+
+Bars (per field, at the point of maximum absolute error, as acVerifyMesh does):
+* FP64 build: acEvalError of the reference (< 5 ulps, or abs error < min|model| * DBL_EPSILON),
+  src/utils/verification.c:78-102.
+* FP32 build against the FP64 oracle: range-normalised max error <= 1e-5 (stated per field).
+* Periodic fill, load/store, MAX/MIN reductions: bitwise.
+* Sums: within 1e-12 (FP64 accumulation) of a compensated sum.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle
+from astaroth_amd import abi
+from astaroth_amd.abi import VTX, RT
+from astaroth_amd.api import Library, HostMesh, real_params_double
+from astaroth_amd.initial import mhd_fields, uniform_interior, seed_for
+
+pytestmark = pytest.mark.gpu
+
+FIELD_NAMES = abi.VTXBUF_NAMES
+
+
+def make_device(lib, info, **kw):
+    kw.setdefault("autotune", False)
+    return lib.device_create(0, info, **kw)
+
+
+@pytest.fixture(params=[False, True], ids=["fp32", "fp64"])
+def lib(request):
+    return Library(request.param)
+
+
+def host_mesh_from(lib, info, fields):
+    return HostMesh(lib, info, [f.astype(lib.dtype) for f in fields])
+
+
+def run_gpu_steps(lib, info, fields, dt, steps, config=None, graph=False):
+    with make_device(lib, info) as dev:
+        if config is not None:
+            dev.set_kernel_config(*config)
+        mesh = host_mesh_from(lib, info, fields)
+        dev.load_mesh(mesh)
+        for _ in range(steps):
+            dev.integrate_step(dt, graph=graph)
+        dev.periodic_boundconds()
+        out = mesh.copy()
+        dev.store_mesh(out)
+        return out.fields
+
+
+def oracle_steps(info, fields, dt, steps, dtype=np.float64):
+    f = [x.astype(dtype) for x in fields]
+    pyoracle.integrate_steps(f, real_params_double(info), dt, steps)
+    pyoracle.periodic(f)
+    return f
+
+
+def test_load_store_roundtrip(lib):
+    info = lib.config(n=24, ny=16, nz=8)
+    rng = np.random.default_rng(0)
+    fields = [rng.standard_normal((14, 22, 30)) for _ in range(8)]
+    mesh = host_mesh_from(lib, info, fields)
+    with make_device(lib, info) as dev:
+        dev.load_mesh(mesh)
+        out = HostMesh(lib, info)
+        dev.store_mesh(out)
+    for a, b in zip(mesh.fields, out.fields):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 16), (40, 24, 12), (7, 9, 5)])
+def test_periodic_fill_bitwise(lib, dims):
+    nx, ny, nz = dims
+    info = lib.config(n=nx, ny=ny, nz=nz)
+    rng = np.random.default_rng(1)
+    fields = [rng.standard_normal((nz + 6, ny + 6, nx + 6)) for _ in range(8)]
+    mesh = host_mesh_from(lib, info, fields)
+    with make_device(lib, info) as dev:
+        dev.load_mesh(mesh)
+        dev.periodic_boundconds()
+        out = HostMesh(lib, info)
+        dev.store_mesh(out)
+        # sub-box fill touches only ghost cells inside [start, end)
+        dev.load_mesh(mesh)
+        start, end = (0, 2, 1), (nx + 4, ny + 6, nz + 2)
+        dev.periodic_boundconds(start, end)
+        part = HostMesh(lib, info)
+        dev.store_mesh(part)
+    for q in range(8):
+        ref = np.pad(mesh.fields[q][3:-3, 3:-3, 3:-3], 3, mode="wrap")
+        assert np.array_equal(out.fields[q], ref)
+        exp = mesh.fields[q].copy()
+        box = (slice(start[2], end[2]), slice(start[1], end[1]), slice(start[0], end[0]))
+        exp[box] = ref[box]
+        assert np.array_equal(part.fields[q], exp)
+
+
+def test_device_uniform_generator_matches_host(lib):
+    nx, ny, nz = 20, 12, 9
+    info = lib.config(n=nx, ny=ny, nz=nz)
+    with make_device(lib, info) as dev:
+        dev.set_vertex_buffer(VTX["VTXBUF_UUY"], 0.0)
+        dev.randomize_uniform(VTX["VTXBUF_UUY"], -1e-3, 1e-3, seed_for(1, 0, 2))
+        dev.synchronize()
+        out = HostMesh(lib, info)
+        dev.store_mesh(out)
+    ref = uniform_interior(nx, ny, nz, -1e-3, 1e-3, seed_for(1, 0, 2)).astype(lib.dtype)
+    assert np.array_equal(out.interior(VTX["VTXBUF_UUY"]), ref)
+
+
+def test_kernel_variants_agree(lib):
+    """Every compiled solve configuration computes the same substeps (variant 0 is the direct
+    stencil evaluation; variant 1 builds cross derivatives by accumulation)."""
+    n = 40
+    info = lib.config(n=n, ny=36, nz=20)
+    fields = mhd_fields(40, 36, 20, np.float64, config_id=5)
+    results = {}
+    for cfg in lib.kernel_configs():
+        for zd in ((0,) if cfg.variant == 0 else (0, 7)):
+            key = (cfg.variant, cfg.tile_x, cfg.tile_y, zd)
+            results[key] = run_gpu_steps(lib, info, fields, 1e-3, 1, config=key)
+    ref = results[(0, 64, 4, 0)]
+    tol = 1e-13 if lib.double else 2e-6
+    for key, res in results.items():
+        for q in range(8):
+            scale = np.max(np.abs(ref[q])) + 1e-300
+            err = np.max(np.abs(res[q].astype(np.float64) - ref[q].astype(np.float64))) / scale
+            assert err <= tol, (key, FIELD_NAMES[q], err)
+
+
+def test_config1_fp64_parity_with_oracle():
+    """Config 1: 64^3, dt = 1e-3, 5 RK3 steps, FP64, acVerifyMesh criterion per field."""
+    lib = Library(True)
+    n = 64
+    info = lib.config(n=n)
+    fields = mhd_fields(n, n, n, np.float64, config_id=1)
+    gpu = run_gpu_steps(lib, info, fields, 1e-3, 5)
+    ref = oracle_steps(info, fields, 1e-3, 5)
+    report = []
+    for q in range(8):
+        v = pyoracle.verify_field(ref[q], gpu[q], 53, np.finfo(np.float64).eps)
+        report.append((FIELD_NAMES[q], v["ulp"], v["abs"], v["ok"]))
+    print("\nconfig-1 FP64 GPU vs oracle (ulp, abs at max-abs point):")
+    for r in report:
+        print("  %-16s %8.3g ulp  %10.3g abs  %s" % r)
+    assert all(r[3] for r in report), report
+
+
+@pytest.mark.parametrize("dims", [(40, 24, 12), (13, 70, 9)])
+def test_odd_grid_fp64_parity_with_oracle(dims):
+    """Non-power-of-two, anisotropic grids with partial tiles: FP64 parity per field."""
+    lib = Library(True)
+    nx, ny, nz = dims
+    info = lib.config(n=nx, ny=ny, nz=nz)
+    fields = mhd_fields(nx, ny, nz, np.float64, config_id=7)
+    gpu = run_gpu_steps(lib, info, fields, 2e-3, 2)
+    ref = oracle_steps(info, fields, 2e-3, 2)
+    for q in range(8):
+        v = pyoracle.verify_field(ref[q], gpu[q], 53, np.finfo(np.float64).eps)
+        assert v["ok"], (FIELD_NAMES[q], v)
+
+
+def test_fp32_error_against_fp64_oracle():
+    """FP32 build vs the FP64 oracle after 5 steps of config 1 (range-normalised, per field)."""
+    lib = Library(False)
+    n = 64
+    info = lib.config(n=n)
+    fields = mhd_fields(n, n, n, np.float64, config_id=1)
+    gpu = run_gpu_steps(lib, info, fields, 1e-3, 5)
+    ref = oracle_steps(Library(True).config(n=n), fields, 1e-3, 5)
+    print("\nconfig-1 FP32 GPU vs FP64 oracle (max |err| / max |model|):")
+    for q in range(8):
+        scale = np.max(np.abs(ref[q]))
+        err = np.max(np.abs(gpu[q].astype(np.float64) - ref[q])) / scale
+        print("  %-16s %.3g" % (FIELD_NAMES[q], err))
+        assert err <= 1e-5, (FIELD_NAMES[q], err)
+
+
+def test_substep_subregion_and_buffers(lib):
+    """acDeviceIntegrateSubstep writes `out` only inside [start, end) and reads `out` as the
+    previous stage for steps 1-2 (integration.cuh:14-37)."""
+    nx, ny, nz = 24, 20, 16
+    info = lib.config(n=nx, ny=ny, nz=nz)
+    fields = mhd_fields(nx, ny, nz, np.float64, config_id=3)
+    prev = [f * 0.5 + 1e-4 for f in fields]
+    start, end = (5, 3, 7), (20, 23, 12)
+    with make_device(lib, info) as dev:
+        m_in = host_mesh_from(lib, info, fields)
+        m_prev = host_mesh_from(lib, info, prev)
+        # full-domain reference substep
+        dev.load_mesh(m_prev)
+        dev.swap_buffers()          # out <- prev
+        dev.load_mesh(m_in)         # in  <- fields
+        assert dev.integrate_substep(1, dt=1e-3) == 0
+        dev.synchronize()
+        dev.swap_buffers()
+        full = HostMesh(lib, info)
+        dev.store_mesh(full)
+        # sub-box substep
+        dev.load_mesh(m_prev)
+        dev.swap_buffers()
+        dev.load_mesh(m_in)
+        assert dev.integrate_substep(1, start, end, dt=1e-3) == 0
+        dev.synchronize()
+        dev.swap_buffers()
+        part = HostMesh(lib, info)
+        dev.store_mesh(part)
+        # regions outside the domain are rejected
+        assert dev.integrate_substep(0, (2, 3, 3), end, dt=1e-3) == abi.AC_FAILURE
+    box = (slice(start[2], end[2]), slice(start[1], end[1]), slice(start[0], end[0]))
+    for q in range(8):
+        exp = m_prev.fields[q].copy()
+        exp[box] = full.fields[q][box]
+        assert np.array_equal(part.fields[q], exp), FIELD_NAMES[q]
+
+
+def test_uniform_state_fixed_point(lib):
+    n = 16
+    info = lib.config(n=n)
+    rng = np.random.default_rng(4)
+    fields = [np.full((n + 6,) * 3, v) for v in rng.uniform(-0.3, 0.3, 8)]
+    gpu = run_gpu_steps(lib, info, fields, 1e-2, 2)
+    for a, b in zip(fields, gpu):
+        assert np.array_equal(a.astype(lib.dtype), b)
+
+
+def test_beltrami_decay(lib):
+    """A Beltrami field decays by the RK3 amplification factor of the discrete Laplacian."""
+    n, k, a0, dt, steps, eta = 32, 2, 0.1, 0.05, 3, 0.3
+    info = lib.config(n=n, AC_eta=eta)
+    h = 2 * math.pi / n
+    m = n + 6
+    zc = ((np.arange(m) - 3) * h)[:, None, None] * np.ones((m, m, m))
+    fields = [np.zeros((m, m, m)) for _ in range(8)]
+    fields[0][...] = 0.3
+    fields[4][...] = a0 * np.sin(k * zc)
+    fields[5][...] = a0 * np.cos(k * zc)
+    gpu = run_gpu_steps(lib, info, fields, dt, steps)
+    k2 = (49.0 / 18.0 - 3 * np.cos(k * h) + 0.3 * np.cos(2 * k * h) - np.cos(3 * k * h) / 45.0) / h**2
+    z = -eta * k2 * dt
+    an = a0 * (1 + z + z**2 / 2 + z**3 / 6) ** steps
+    tol = 1e-13 if lib.double else 2e-6
+    sl = (slice(3, -3),) * 3
+    assert np.max(np.abs(gpu[4][sl] - an * np.sin(k * zc[sl]))) < tol
+    assert np.max(np.abs(gpu[5][sl] - an * np.cos(k * zc[sl]))) < tol
+
+
+def test_graph_step_matches_eager(lib):
+    """acDeviceIntegrateStepGraph replays give the same state as eager steps, replay after
+    replay, with eager work and synchronisation in between."""
+    n = 24
+    info = lib.config(n=n)
+    fields = mhd_fields(n, n, n, np.float64, config_id=2)
+    eager = run_gpu_steps(lib, info, fields, 1e-3, 4)
+    with make_device(lib, info) as dev:
+        mesh = host_mesh_from(lib, info, fields)
+        dev.load_mesh(mesh)
+        for i in range(4):
+            dev.integrate_step(1e-3, graph=True)
+            dev.synchronize()
+            dev.reduce_scal("RTYPE_MAX", "VTXBUF_LNRHO")  # eager work between replays
+        dev.periodic_boundconds()
+        out = mesh.copy()
+        dev.store_mesh(out)
+    for a, b in zip(eager, out.fields):
+        assert np.array_equal(a, b)
+
+
+def test_reductions_against_oracle(lib):
+    nx, ny, nz = 33, 20, 17
+    info = lib.config(n=nx, ny=ny, nz=nz)
+    fields = mhd_fields(nx, ny, nz, np.float64, config_id=4)
+    fields = [f.astype(lib.dtype) for f in fields]
+    sl = (slice(3, -3),) * 3
+    with make_device(lib, info) as dev:
+        dev.load_mesh(host_mesh_from(lib, info, fields))
+        lr, ux, uy, uz = fields[0], fields[1], fields[2], fields[3]
+        # MAX/MIN: bitwise against the oracle (order independent)
+        for rt in ("RTYPE_MAX", "RTYPE_MIN"):
+            assert dev.reduce_scal(rt, "VTXBUF_LNRHO") == pyoracle.reduce([lr], rt)
+            assert dev.reduce_vec(rt, "VTXBUF_UUX", "VTXBUF_UUY", "VTXBUF_UUZ") == \
+                pyoracle.reduce([ux, uy, uz], rt)
+        # sums: device level returns the raw sum (the caller applies sqrt(sum / n^3))
+        a = lr[sl].astype(np.float64)
+        exact = math.fsum((a * a).ravel())
+        assert dev.reduce_scal("RTYPE_RMS", "VTXBUF_LNRHO", double=True) == pytest.approx(exact, rel=1e-12)
+        exact = math.fsum(np.exp(2 * a).ravel())
+        assert dev.reduce_scal("RTYPE_RMS_EXP", "VTXBUF_LNRHO", double=True) == pytest.approx(exact, rel=1e-12)
+        exact = math.fsum(a.ravel())
+        assert dev.reduce_scal("RTYPE_SUM", "VTXBUF_LNRHO", double=True) == pytest.approx(exact, rel=1e-9, abs=1e-12)
+        u = [x[sl].astype(np.float64) for x in (ux, uy, uz)]
+        L2 = u[0] ** 2 + u[1] ** 2 + u[2] ** 2
+        assert dev.reduce_vec("RTYPE_RMS", "VTXBUF_UUX", "VTXBUF_UUY", "VTXBUF_UUZ", double=True) == \
+            pytest.approx(math.fsum(L2.ravel()), rel=1e-12)
+        assert dev.reduce_vec("RTYPE_SUM", "VTXBUF_UUX", "VTXBUF_UUY", "VTXBUF_UUZ", double=True) == \
+            pytest.approx(math.fsum(np.sqrt(L2).ravel()), rel=1e-12)
+        # AcReal-typed result is the FP64 value rounded once
+        r = dev.reduce_scal("RTYPE_RMS", "VTXBUF_LNRHO")
+        assert r == float(lib.dtype(dev.reduce_scal("RTYPE_RMS", "VTXBUF_LNRHO", double=True)))
+        # Alfven speed |B| / sqrt(4 pi exp(lnrho)) with B taken from three fields
+        va = np.sqrt(L2) / np.sqrt(4 * math.pi * np.exp(a))
+        assert dev.reduce_vec_scal("RTYPE_ALFVEN_MAX", "VTXBUF_UUX", "VTXBUF_UUY", "VTXBUF_UUZ",
+                                   "VTXBUF_LNRHO", double=True) == pytest.approx(va.max(), rel=1e-14)
+        assert dev.reduce_vec_scal("RTYPE_ALFVEN_RMS", "VTXBUF_UUX", "VTXBUF_UUY", "VTXBUF_UUZ",
+                                   "VTXBUF_LNRHO", double=True) == pytest.approx(math.fsum((va**2).ravel()), rel=1e-12)
+        # deterministic: the same call twice gives the same bits
+        assert dev.reduce_scal("RTYPE_SUM", "VTXBUF_UUX", double=True) == \
+            dev.reduce_scal("RTYPE_SUM", "VTXBUF_UUX", double=True)
+
+
+def test_uniform_loaders_reject_invalid_values(lib):
+    info = lib.config(n=8)
+    with make_device(lib, info) as dev:
+        assert dev.load_uniform("AC_nu_visc", float("nan")) == abi.AC_FAILURE
+        assert dev.load_uniform("AC_nu_visc", 1e-3) == abi.AC_SUCCESS
+        assert lib.dll.acDeviceLoadScalarUniform(dev.handle, 0, 999, 1.0) == abi.AC_FAILURE
+        assert lib.dll.acDeviceSynchronizeStream(dev.handle, 77) == abi.AC_FAILURE
+        assert lib.dll.acDeviceGeneralBoundconds(dev.handle, 0, abi.int3(0, 0, 0), abi.int3(14, 14, 14),
+                                                 info, abi.int3(0, 0, 0)) == abi.AC_FAILURE
+
+
+def test_autotune_selects_a_valid_config():
+    lib = Library(False)
+    n = 48
+    info = lib.config(n=n)
+    fields = mhd_fields(n, n, n, np.float64, config_id=1)
+    with make_device(lib, info, autotune=True) as dev:
+        cfg = dev.kernel_config()
+        assert cfg.variant in (0, 1) and cfg.tile_x > 0
+        mesh = host_mesh_from(lib, info, fields)
+        dev.load_mesh(mesh)
+        dev.integrate_step(1e-3)
+        dev.periodic_boundconds()
+        out = mesh.copy()
+        dev.store_mesh(out)
+    ref = oracle_steps(Library(True).config(n=n), fields, 1e-3, 1)
+    for q in range(8):
+        scale = np.max(np.abs(ref[q]))
+        assert np.max(np.abs(out.fields[q].astype(np.float64) - ref[q])) / scale < 1e-5
