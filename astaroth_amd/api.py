@@ -121,6 +121,7 @@ class Library:
             "acDeviceTimeKernelConfig": (ctypes.c_float, [vp, abi.KernelConfig, i]),
             "acDeviceRandomizeUniform": (i, [vp, i, i, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong]),
             "acDeviceGetStreamHandle": (vp, [vp, i]),
+            "acDeviceCopyVertexBuffer": (i, [vp, i, i]),
             "acHostUpdateBuiltinParams": (i, [ctypes.POINTER(MI)]),
             "acHostMeshCreate": (i, [MI, ctypes.POINTER(M)]),
             "acHostMeshRandomize": (i, [ctypes.POINTER(M)]),
@@ -359,6 +360,10 @@ class Device:
 
     def autotune(self):
         return self._check(self.lib.dll.acDeviceAutoOptimize(self.handle), "acDeviceAutoOptimize")
+
+    def copy_vertex_buffer(self, handle, stream=abi.STREAM_DEFAULT):
+        return self._check(self.lib.dll.acDeviceCopyVertexBuffer(self.handle, stream, _vh(handle)),
+                           "acDeviceCopyVertexBuffer")
 
     def stream_handle(self, stream=abi.STREAM_DEFAULT):
         return self.lib.dll.acDeviceGetStreamHandle(self.handle, stream)

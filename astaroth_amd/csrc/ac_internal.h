@@ -101,6 +101,8 @@ int reduce_partials_needed();
 hipError_t launch_fill_uniform(AcReal* field, const AcMeshInfo& info, double lo, double hi,
                                unsigned long long seed, hipStream_t stream);
 hipError_t launch_fill_constant(AcReal* field, long n, AcReal value, hipStream_t stream);
+// Plain element copy (one read and one write per element).
+hipError_t launch_copy(AcReal* dst, const AcReal* src, long n, hipStream_t stream);
 
 } // namespace ac
 

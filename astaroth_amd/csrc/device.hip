@@ -423,7 +423,11 @@ acDeviceCreate(const int id, const AcMeshInfo config, Device* device_handle)
         return AC_FAILURE;
     }
     int count = 0;
-    AC_HIP_CHECK(hipGetDeviceCount(&count));
+    if (hipGetDeviceCount(&count) != hipSuccess) {
+        (void)hipGetLastError();
+        fprintf(stderr, "acDeviceCreate: no HIP device is available\n");
+        return AC_FAILURE;
+    }
     if (id < 0 || id >= count) {
         fprintf(stderr, "acDeviceCreate: no GPU with id %d (%d visible)\n", id, count);
         return AC_FAILURE;
@@ -955,6 +959,17 @@ acDeviceRandomizeUniform(const Device device, const Stream stream, const VertexB
     AC_HIP_CHECK(hipSetDevice(device->id));
     AC_HIP_CHECK(launch_fill_uniform(device->vba.in[handle], device->local_config, lo, hi, seed,
                                      device->streams[stream]));
+    return AC_SUCCESS;
+}
+
+AcResult
+acDeviceCopyVertexBuffer(const Device device, const Stream stream, const VertexBufferHandle handle)
+{
+    if (!stream_ok(device, stream) || handle < 0 || handle >= NF)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(device->id));
+    AC_HIP_CHECK(launch_copy(device->vba.out[handle], device->vba.in[handle],
+                             (long)acVertexBufferSize(device->local_config), device->streams[stream]));
     return AC_SUCCESS;
 }
 

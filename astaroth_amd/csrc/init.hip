@@ -23,6 +23,7 @@ __global__ void __launch_bounds__(256)
 fill_uniform_kernel(AcReal* f, int nx, int ny, int nz, int mx, int mxy, double lo, double hi,
                     unsigned long long seed)
 {
+#pragma clang fp contract(off) // lo + (hi - lo) u rounds exactly as on the host
     const long t     = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long total = (long)nx * ny * nz;
     if (t >= total)
@@ -44,7 +45,25 @@ fill_constant_kernel(AcReal* f, long n, AcReal value)
         f[t] = value;
 }
 
+__global__ void __launch_bounds__(256)
+copy_kernel(AcReal* __restrict__ dst, const AcReal* __restrict__ src, long n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n)
+        dst[t] = src[t];
+}
+
 } // namespace
+
+hipError_t
+launch_copy(AcReal* dst, const AcReal* src, long n, hipStream_t stream)
+{
+    const long blocks = (n + 255) / 256;
+    if (blocks <= 0)
+        return hipSuccess;
+    copy_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(dst, src, n);
+    return hipGetLastError();
+}
 
 hipError_t
 launch_fill_uniform(AcReal* field, const AcMeshInfo& info, double lo, double hi,

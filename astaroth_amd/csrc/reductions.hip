@@ -46,6 +46,8 @@ identity(int op)
 __device__ __forceinline__ double
 filter(const RedArgs& a, long idx)
 {
+    // No FMA contraction: MAX/MIN of vector lengths must round like the host's a*a + b*b + c*c.
+#pragma clang fp contract(off)
     const double kFourPi = 12.566370614359172953850573533118;
     if (a.nvals == 1) {
         const AcReal x = a.v[0][idx];

@@ -1,0 +1,286 @@
+#!/usr/bin/env python3
+"""Benchmark: cell-updates/s per RK3 step of the 8-field MHD solver at 512^3, FP32 and FP64.
+
+One step = 3 x {periodic halo fill, fused solve + RK3 substep, buffer swap} over the whole box,
+driven through the C ABI (acDevicePeriodicBoundconds, acDeviceIntegrateSubstep,
+acDeviceSwapBuffers), the caller sequence of the reference's single-GPU step
+(src/core/node.cc:729-806). Inputs are generated on the GPU (seeded splitmix64, the config-1
+distribution) and stay resident in HBM; the timed region is K steps bracketed by device
+synchronisation. `value` is the FP32 rate; the FP64 build is timed on the same workload.
+
+Roofline of the dominant kernel (the solve): algorithmic bytes per substep launch are
+16 n^3 s (substep 0: read 8 fields, write 8) or 24 n^3 s (substeps 1-2: also read the previous
+stage), 64 n^3 s per step; achieved = those bytes / the solve launches' mean duration, measured
+with HIP events recorded on the library's stream around every solve launch in the timed loop.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import statistics
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "cell-updates/s per RK3 step at 512^3, FP32 vs FP64, + % of roofline"
+HBM_PEAK_GBS = 8000.0
+VALU_FP32_PEAK_TFLOPS = 157.3  # vendor figure (counts packed FP32; packed math is not used here)
+VALU_FP64_PEAK_TFLOPS = 78.6
+
+
+class Hip:
+    def __init__(self):
+        self.dll = ctypes.CDLL("libamdhip64.so")
+        self.dll.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.dll.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.dll.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.dll.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.dll.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def event(self):
+        ev = ctypes.c_void_p()
+        if self.dll.hipEventCreate(ctypes.byref(ev)) != 0:
+            raise RuntimeError("hipEventCreate failed")
+        return ev
+
+    def record(self, ev, stream):
+        if self.dll.hipEventRecord(ev, stream) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
+    def elapsed(self, a, b):
+        ms = ctypes.c_float()
+        self.dll.hipEventSynchronize(b)
+        if self.dll.hipEventElapsedTime(ctypes.byref(ms), a, b) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return ms.value
+
+    def sync(self):
+        self.dll.hipDeviceSynchronize()
+
+
+def run_gpu(double: bool, n: int, steps: int, warmup: int, dt: float, autotune: bool, hip: Hip,
+            kernel_config=None):
+    from astaroth_amd.api import Library
+    from astaroth_amd.initial import device_init_mhd
+
+    lib = Library(double)
+    info = lib.config(n=n)
+    t0 = time.time()
+    dev = lib.device_create(0, info, autotune=autotune)
+    tune_s = time.time() - t0
+    try:
+        if kernel_config is not None:
+            dev.set_kernel_config(*kernel_config)
+        device_init_mhd(dev, config_id=1)
+        stream = dev.stream_handle()
+        nmin, nmax = (3, 3, 3), (n + 3, n + 3, n + 3)
+        for _ in range(warmup):
+            dev.integrate_step(dt)
+        dev.synchronize()
+        evs = [[hip.event() for _ in range(6)] for _ in range(steps)]
+        ev_begin, ev_end = hip.event(), hip.event()
+        d = lib.dll
+        from astaroth_amd.abi import int3
+
+        zero, m = int3(0, 0, 0), int3(n + 6, n + 6, n + 6)
+        s_min, s_max = int3(*nmin), int3(*nmax)
+        hip.sync()
+        wall0 = time.perf_counter()
+        hip.record(ev_begin, stream)
+        for k in range(steps):
+            e = evs[k]
+            for s in range(3):
+                d.acDevicePeriodicBoundconds(dev.handle, 0, zero, m)
+                hip.record(e[2 * s], stream)
+                d.acDeviceIntegrateSubstep(dev.handle, 0, s, s_min, s_max, lib.real(dt))
+                hip.record(e[2 * s + 1], stream)
+                d.acDeviceSwapBuffers(dev.handle)
+        hip.record(ev_end, stream)
+        hip.sync()
+        wall = time.perf_counter() - wall0
+        total_ms = hip.elapsed(ev_begin, ev_end)
+        solve_ms = [hip.elapsed(e[2 * s], e[2 * s + 1]) for e in evs for s in range(3)]
+        step_ms = []
+        prev = ev_begin
+        for e in evs:
+            step_ms.append(hip.elapsed(prev, e[5]))
+            prev = e[5]
+        cfg = dev.kernel_config()
+        # sanity: the state stays finite
+        mx = dev.reduce_scal("RTYPE_MAX", "VTXBUF_LNRHO", double=True)
+        if not (mx == mx) or abs(mx) > 1.0:
+            raise RuntimeError("state diverged: max lnrho = %r" % mx)
+    finally:
+        dev.destroy()
+    s = 8 if double else 4
+    cells = float(n) ** 3
+    ms_per_step = total_ms / steps
+    solve_mean = sum(solve_ms) / len(solve_ms)
+    alg_bytes_per_launch = 64.0 * cells * s / 3.0  # mean over substeps 0, 1, 2
+    achieved = alg_bytes_per_launch / (solve_mean * 1e-3) / 1e9
+    return dict(
+        ms_per_step=ms_per_step,
+        value=cells / (ms_per_step * 1e-3),
+        median_step_ms=statistics.median(step_ms),
+        p90_step_ms=sorted(step_ms)[min(len(step_ms) - 1, int(0.9 * len(step_ms)))],
+        solve_ms_mean=solve_mean,
+        solve_ms_by_substep=[sum(solve_ms[i::3]) / steps for i in range(3)],
+        solve_fraction=sum(solve_ms) / total_ms,
+        achieved_gbs=achieved,
+        alg_bytes_per_launch=alg_bytes_per_launch,
+        wall_s=wall,
+        autotune_s=tune_s,
+        kernel_config=dict(variant=cfg.variant, tile_x=cfg.tile_x, tile_y=cfg.tile_y, z_depth=cfg.z_depth),
+    )
+
+
+def copy_probe(double: bool, n: int, reps: int, hip: Hip):
+    """Known-size copy kernel (acDeviceCopyVertexBuffer: one read and one write of m^3 cells per
+    launch) for calibrating memory counters and as the measured copy bandwidth."""
+    from astaroth_amd.api import Library
+
+    lib = Library(double)
+    info = lib.config(n=n)
+    dev = lib.device_create(0, info, autotune=False)
+    try:
+        stream = dev.stream_handle()
+        dev.copy_vertex_buffer(0)
+        dev.synchronize()
+        a, b = hip.event(), hip.event()
+        hip.record(a, stream)
+        for r in range(reps):
+            dev.copy_vertex_buffer(r % 8)
+        hip.record(b, stream)
+        ms = hip.elapsed(a, b) / reps
+    finally:
+        dev.destroy()
+    nbytes = 2.0 * (n + 6) ** 3 * (8 if double else 4)
+    return dict(bytes_per_launch=nbytes, ms_per_launch=ms, gbs=nbytes / (ms * 1e-3) / 1e9)
+
+
+def measured_traffic(prec: str, n: int, cfg: dict):
+    """HBM bytes per solve launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summaries
+    (profiles/*_hbm_traffic.md, counters scaled by a copy-kernel calibration), when one exists for
+    this precision, grid and kernel configuration; otherwise None."""
+    import glob
+    import re
+
+    key = "%d,%d,%d,%d" % (cfg["variant"], cfg["tile_x"], cfg["tile_y"], cfg["z_depth"])
+    for path in sorted(glob.glob(os.path.join(REPO, "profiles", "*_%dcubed_hbm_traffic.md" % n))):
+        for line in open(path):
+            m = re.match(r"traffic_(fp32|fp64) (\S+) bytes_per_launch=(\S+)", line.strip())
+            if m and m.group(1) == prec and m.group(2) == key:
+                return float(m.group(3)), os.path.relpath(path, REPO)
+    return None, None
+
+
+def cpu_baseline(n=128, steps=3):
+    """The oracle (CPU restatement of the host model solver, FP64, OpenMP over k) on the host."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import numpy as np
+    import pyoracle
+    from astaroth_amd.api import Library, real_params_double
+    from astaroth_amd.initial import mhd_fields
+
+    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16)
+    threads = pyoracle.set_threads(threads)
+    info = Library(True).config(n=n)
+    fields = mhd_fields(n, n, n, np.float64, config_id=1)
+    rp = real_params_double(info)
+    pyoracle.integrate_steps([f.copy() for f in fields], rp, 1e-3, 1)  # warm
+    t0 = time.perf_counter()
+    pyoracle.integrate_steps(fields, rp, 1e-3, steps)
+    dt = time.perf_counter() - t0
+    return dict(value=n**3 * steps / dt, unit="cell-updates/s", cores=threads, kind="port",
+                sample="%d^3 config-1 box, %d RK3 step(s), FP64 host-form model solver, %.2f s" % (n, steps, dt))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--dt", type=float, default=1e-3)
+    ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-config", default=None,
+                    help="force a solve configuration 'variant,tile_x,tile_y,z_depth' instead of autotuning")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="REPS",
+                    help="only run REPS launches of the plain copy kernel and print its bandwidth")
+    a = ap.parse_args(argv)
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this path has no multi-GPU layer (single periodic box "
+              "per device); run it with --gpus 1", file=sys.stderr)
+        return 2
+    hip = Hip()
+    if a.copy_probe:
+        for prec in (["fp32", "fp64"] if a.precision == "both" else [a.precision]):
+            print(json.dumps(dict(copy_probe=prec, n=a.n, **copy_probe(prec == "fp64", a.n, a.copy_probe, hip))),
+                  flush=True)
+        return 0
+    kcfg = tuple(int(v) for v in a.kernel_config.split(",")) if a.kernel_config else None
+    res = {}
+    for prec in (["fp32", "fp64"] if a.precision == "both" else [a.precision]):
+        res[prec] = run_gpu(prec == "fp64", a.n, a.steps if prec == "fp32" else max(3, a.steps // 2),
+                            a.warmup, a.dt, not a.no_autotune and kcfg is None, hip, kcfg)
+        print("# %s: %.3f ms/step, %.4g cell-updates/s, solve %.3f ms/launch, %.0f GB/s, config %s"
+              % (prec, res[prec]["ms_per_step"], res[prec]["value"], res[prec]["solve_ms_mean"],
+                 res[prec]["achieved_gbs"], res[prec]["kernel_config"]), file=sys.stderr, flush=True)
+    main_prec = "fp32" if "fp32" in res else "fp64"
+    r = res[main_prec]
+    traffic, traffic_src = measured_traffic(main_prec, a.n, r["kernel_config"])
+    line = {
+        "metric": METRIC,
+        "value": r["value"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if main_prec == "fp32" else "f64",
+        "data": "synthetic: seeded splitmix64 config-1 fields (lnrho~U(-1e-2,1e-2), u,A~U(-1e-3,1e-3), s=0)",
+        "config": {
+            "workload": "8-field compressible MHD, %d^3 periodic box, 6th-order stencils, 2N-RK3, dt=%g" % (a.n, a.dt),
+            "grid": [a.n, a.n, a.n],
+            "step": "3 x {periodic fill, solve+RK3 substep, swap}",
+            "kernel_config": r["kernel_config"],
+            "median_step_ms": r["median_step_ms"],
+            "p90_step_ms": r["p90_step_ms"],
+            "solve_fraction_of_step": r["solve_fraction"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "solve (fused MHD stencil + RK3 substep)",
+            "achieved": r["achieved_gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": r["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
+            "alg_bytes_per_launch": r["alg_bytes_per_launch"],
+            "solve_ms_per_launch": r["solve_ms_mean"],
+        },
+    }
+    if "fp64" in res and "fp32" in res:
+        f64 = res["fp64"]
+        line["fp64"] = {"value": f64["value"], "ms_per_step": f64["ms_per_step"],
+                        "achieved_gbs": f64["achieved_gbs"], "frac": f64["achieved_gbs"] / HBM_PEAK_GBS,
+                        "kernel_config": f64["kernel_config"], "solve_ms_per_launch": f64["solve_ms_mean"]}
+        line["fp32_over_fp64"] = r["value"] / f64["value"]
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

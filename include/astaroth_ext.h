@@ -64,6 +64,11 @@ AcResult acDeviceRandomizeUniform(const Device device, const Stream stream,
                                   const VertexBufferHandle handle, const double lo,
                                   const double hi, const unsigned long long seed);
 
+/* ---- Plain copy of in[handle] into out[handle] (all m^3 cells), asynchronous on `stream`.
+ * One read and one write per cell: the known-size kernel used to calibrate memory counters. */
+AcResult acDeviceCopyVertexBuffer(const Device device, const Stream stream,
+                                  const VertexBufferHandle handle);
+
 /* ---- Stream access for timing from the caller (returns the hipStream_t as void*) ---- */
 void* acDeviceGetStreamHandle(const Device device, const Stream stream);
 

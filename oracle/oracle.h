@@ -11,13 +11,16 @@
  *   src/utils/modelreduce.c   host reductions
  *   src/utils/memory.c        periodic ghost fill
  *   src/core/astaroth.cc      builtin parameters (m = n + 6, n_min, n_max, inv_ds)
+ *   src/core/kernels/integration.cuh  two-buffer RK3 substep form of the GPU path
  *
  * Pinning status (what checks this restatement, see tests/test_oracle_*.py):
  *   FD operators          closed forms on sin(kx) and exactness on polynomials
  *   equations (RHS)       polynomial fields (FD exact) against an independent analytic
  *                         formulation (pressure as -grad(p)/rho, viscosity as
  *                         div(2 rho nu S)/rho, current as curl curl A)
- *   RK3                   Beltrami decay against the amplification factor 1+z+z^2/2+z^3/6
+ *   RK3 (host form and the GPU path's two-buffer form)
+ *                         Beltrami decay against the amplification factor 1+z+z^2/2+z^3/6;
+ *                         the two forms agree to rounding with each other and with long double
  *   periodic fill         numpy.pad(mode="wrap"), bitwise
  *   reductions            numpy max/min and compensated (math.fsum) sums
  *   error metric          known-answer values of ulp/abs/rel error
@@ -382,6 +385,56 @@ void integrate_step(const Grid<T>& g, T* const* f, const Params<T>& p, T dt)
                         f[q][idx] += beta[s] * w[q][idx];
                 }
     }
+}
+
+// ---- Two-buffer RK3 substep of the reference's GPU path (src/core/kernels/integration.cuh:14-37,
+// launched over [start, end) by integration.cuh:217-249 and device.cc:427-434) ----
+// out = in + beta_0 F dt                                       (step 0)
+// out = in + beta_s (alpha_s (1 / beta_{s-1}) (in - out) + F dt)  (steps 1, 2)
+// `out` holds the previous stage on entry; ghost cells of `in` must be filled by the caller.
+template <typename T>
+void gpu_form_substep(const Grid<T>& g, T* const* in, T* const* out, const Params<T>& p, int step,
+                      T dt, const int* start, const int* end)
+{
+    const T alpha[] = {T(0), T(.0), T(-5. / 9.), T(-153. / 128.)};
+    const T beta[]  = {T(0), T(1. / 3.), T(15. / 16.), T(8. / 15.)};
+#pragma omp parallel for schedule(static)
+    for (int k = start[2]; k < end[2]; ++k)
+        for (int j = start[1]; j < end[1]; ++j)
+            for (int i = start[0]; i < end[0]; ++i) {
+                T rate[NFIELDS];
+                rhs_cell(g, in, i, j, k, p, rate);
+                const size_t idx = g.idx(i, j, k);
+                for (int q = 0; q < NFIELDS; ++q) {
+                    const T cur = in[q][idx];
+                    if (step == 0)
+                        out[q][idx] = cur + beta[step + 1] * rate[q] * dt;
+                    else
+                        out[q][idx] = cur + beta[step + 1] * (alpha[step + 1] * (T(1.) / beta[step]) *
+                                                                  (cur - out[q][idx]) +
+                                                              rate[q] * dt);
+                }
+            }
+}
+
+// One RK3 step driven the way a single-device caller drives the GPU path:
+// 3 x {periodic fill of in, substep over the interior, swap}. Returns the index (0 or 1) of the
+// buffer set holding the new state.
+template <typename T>
+int gpu_form_step(const Grid<T>& g, T* const* a, T* const* b, const Params<T>& p, T dt)
+{
+    const int start[3] = {NGHOST_, NGHOST_, NGHOST_};
+    const int end[3]   = {NGHOST_ + g.nx, NGHOST_ + g.ny, NGHOST_ + g.nz};
+    T* const* in       = a;
+    T* const* out      = b;
+    for (int s = 0; s < 3; ++s) {
+        apply_periodic(g, in);
+        gpu_form_substep(g, in, out, p, s, dt, start, end);
+        T* const* t = in;
+        in          = out;
+        out         = t;
+    }
+    return in == a ? 0 : 1;
 }
 
 // ---- Host reductions (modelreduce.c:76-209) ----

@@ -64,6 +64,31 @@ void derivs_cell(const int* dims, const T* field, const double* inv_ds, int i, i
         for (int s = 0; s < nsteps; ++s)                                                           \
             integrate_step(g, fields, p, dt);                                                      \
     }                                                                                              \
+    extern "C" void oracle_integrate_steps_gpuform_##SFX(const int* dims, T* const* fields,     \
+                                                         T* const* scratch, const double* rp,   \
+                                                         T dt, int nsteps)                       \
+    {                                                                                              \
+        const Grid<T> g(dims[0], dims[1], dims[2]);                                                \
+        const Params<T> p = make_params<T>(rp);                                                    \
+        T* const* a = fields;                                                                      \
+        T* const* b = scratch;                                                                     \
+        for (int s = 0; s < nsteps; ++s)                                                           \
+            if (gpu_form_step(g, a, b, p, dt) == 1) {                                              \
+                T* const* t = a;                                                                   \
+                a           = b;                                                                   \
+                b           = t;                                                                   \
+            }                                                                                      \
+        if (a != fields)                                                                           \
+            for (int q = 0; q < NFIELDS; ++q)                                                      \
+                memcpy(fields[q], a[q], sizeof(T) * g.size());                                     \
+    }                                                                                              \
+    extern "C" void oracle_substep_gpuform_##SFX(const int* dims, T* const* in, T* const* out,   \
+                                                 const double* rp, int step, T dt,                 \
+                                                 const int* start, const int* end)                 \
+    {                                                                                              \
+        const Grid<T> g(dims[0], dims[1], dims[2]);                                                \
+        gpu_form_substep(g, in, out, make_params<T>(rp), step, dt, start, end);                    \
+    }                                                                                              \
     extern "C" void oracle_rhs_##SFX(const int* dims, T* const* fields, const double* rp,        \
                                      T* const* rates)                                              \
     {                                                                                              \

@@ -66,6 +66,27 @@ def integrate_steps(fields, real_params, dt, nsteps=1):
     return fields
 
 
+def integrate_steps_gpu_form(fields, real_params, dt, nsteps=1):
+    """nsteps RK3 steps in the two-buffer form of the reference's GPU path (integration.cuh:14-37),
+    driven as 3 x {periodic fill, substep, swap}; the new state is written back into `fields`."""
+    t = _key(fields[0].dtype)
+    fn = getattr(lib(), "oracle_integrate_steps_gpuform_" + _SFX[t])
+    scratch = [np.zeros_like(f) for f in fields]
+    rp, prp = _rp(real_params)
+    fn(_dims(fields), _ptrs(fields, fields[0].dtype), _ptrs(scratch, fields[0].dtype), prp, _CT[t](dt), int(nsteps))
+    return fields
+
+
+def substep_gpu_form(in_fields, out_fields, real_params, step, dt, start, end):
+    """acDeviceIntegrateSubstep semantics on the CPU: writes out_fields inside [start, end)."""
+    t = _key(in_fields[0].dtype)
+    fn = getattr(lib(), "oracle_substep_gpuform_" + _SFX[t])
+    rp, prp = _rp(real_params)
+    fn(_dims(in_fields), _ptrs(in_fields, in_fields[0].dtype), _ptrs(out_fields, in_fields[0].dtype), prp,
+       int(step), _CT[t](dt), (ctypes.c_int * 3)(*start), (ctypes.c_int * 3)(*end))
+    return out_fields
+
+
 def rhs(fields, real_params):
     """Rate of change of the 8 fields at every interior cell (no ghost fill)."""
     t = _key(fields[0].dtype)

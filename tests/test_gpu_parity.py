@@ -53,11 +53,27 @@ def run_gpu_steps(lib, info, fields, dt, steps, config=None, graph=False):
         return out.fields
 
 
-def oracle_steps(info, fields, dt, steps, dtype=np.float64):
+def oracle_steps(info, fields, dt, steps, dtype=np.float64, gpu_form=False):
+    """The oracle's RK3 steps: host form (modelsolver.c) or the GPU path's two-buffer form
+    (integration.cuh:14-37), with a final periodic fill like the device comparison."""
     f = [x.astype(dtype) for x in fields]
-    pyoracle.integrate_steps(f, real_params_double(info), dt, steps)
+    if gpu_form:
+        pyoracle.integrate_steps_gpu_form(f, real_params_double(info), dt, steps)
+    else:
+        pyoracle.integrate_steps(f, real_params_double(info), dt, steps)
     pyoracle.periodic(f)
     return f
+
+
+def verify_all(ref, cand, label):
+    out = []
+    for q in range(8):
+        v = pyoracle.verify_field(np.asarray(ref[q], dtype=np.float64), cand[q], 53, np.finfo(np.float64).eps)
+        out.append(v)
+    print("\n%s (ulp, abs at the max-abs-error point):" % label)
+    for q, v in enumerate(out):
+        print("  %-16s %8.3g ulp  %10.3g abs  %s" % (FIELD_NAMES[q], v["ulp"], v["abs"], "OK" if v["ok"] else "FAIL"))
+    return out
 
 
 def test_load_store_roundtrip(lib):
@@ -134,21 +150,28 @@ def test_kernel_variants_agree(lib):
 
 
 def test_config1_fp64_parity_with_oracle():
-    """Config 1: 64^3, dt = 1e-3, 5 RK3 steps, FP64, acVerifyMesh criterion per field."""
+    """Config 1: 64^3, dt = 1e-3, 5 RK3 steps, FP64, acVerifyMesh criterion per field.
+
+    Gate: acEvalError against the oracle's restatement of the same (two-buffer) RK3 form in FP64.
+    Reported: against the host model solver in long double (sanity bound 8 ulp) and in FP64. The
+    host form stores w explicitly while the GPU path reconstructs w from (in - out) / beta; that
+    difference of algorithm alone is 4-5 ulp at the max-abs point here, the same as the oracle
+    reproduces between its two forms on the CPU."""
     lib = Library(True)
     n = 64
     info = lib.config(n=n)
     fields = mhd_fields(n, n, n, np.float64, config_id=1)
     gpu = run_gpu_steps(lib, info, fields, 1e-3, 5)
-    ref = oracle_steps(info, fields, 1e-3, 5)
-    report = []
+    same_form = verify_all(oracle_steps(info, fields, 1e-3, 5, gpu_form=True), gpu,
+                           "config-1 FP64 GPU vs oracle, same RK3 form, FP64")
+    exact = verify_all(oracle_steps(info, fields, 1e-3, 5, dtype=np.longdouble), gpu,
+                       "config-1 FP64 GPU vs host model solver in long double")
+    verify_all(oracle_steps(info, fields, 1e-3, 5), gpu,
+               "config-1 FP64 GPU vs host model solver in FP64 (reported)")
     for q in range(8):
-        v = pyoracle.verify_field(ref[q], gpu[q], 53, np.finfo(np.float64).eps)
-        report.append((FIELD_NAMES[q], v["ulp"], v["abs"], v["ok"]))
-    print("\nconfig-1 FP64 GPU vs oracle (ulp, abs at max-abs point):")
-    for r in report:
-        print("  %-16s %8.3g ulp  %10.3g abs  %s" % r)
-    assert all(r[3] for r in report), report
+        assert same_form[q]["ok"] and same_form[q]["ulp"] <= 3, (FIELD_NAMES[q], same_form[q])
+        # the FP64 two-buffer form itself carries a few ulps of rounding against long double
+        assert exact[q]["ulp"] <= 8, (FIELD_NAMES[q], exact[q])
 
 
 @pytest.mark.parametrize("dims", [(40, 24, 12), (13, 70, 9)])
@@ -159,10 +182,44 @@ def test_odd_grid_fp64_parity_with_oracle(dims):
     info = lib.config(n=nx, ny=ny, nz=nz)
     fields = mhd_fields(nx, ny, nz, np.float64, config_id=7)
     gpu = run_gpu_steps(lib, info, fields, 2e-3, 2)
-    ref = oracle_steps(info, fields, 2e-3, 2)
+    res = verify_all(oracle_steps(info, fields, 2e-3, 2, gpu_form=True), gpu, "odd grid %s" % (dims,))
     for q in range(8):
-        v = pyoracle.verify_field(ref[q], gpu[q], 53, np.finfo(np.float64).eps)
-        assert v["ok"], (FIELD_NAMES[q], v)
+        assert res[q]["ok"] and res[q]["ulp"] <= 3, (FIELD_NAMES[q], res[q])
+
+
+@pytest.mark.parametrize("step", [0, 1, 2])
+def test_substep_matches_oracle_substep(lib, step):
+    """acDeviceIntegrateSubstep over a sub-box against the oracle's restatement of one substep
+    of the GPU path, with arbitrary previous-stage data in `out`."""
+    nx, ny, nz = 30, 22, 18
+    info = lib.config(n=nx, ny=ny, nz=nz)
+    fields = mhd_fields(nx, ny, nz, np.float64, config_id=9)
+    prev = [f * 0.75 - 2e-4 for f in fields]
+    start, end = (3, 6, 4), (33, 20, 21)
+    with make_device(lib, info) as dev:
+        dev.load_mesh(host_mesh_from(lib, info, prev))
+        dev.swap_buffers()
+        dev.load_mesh(host_mesh_from(lib, info, fields))
+        assert dev.integrate_substep(step, start, end, dt=2e-3) == 0
+        dev.synchronize()
+        dev.swap_buffers()
+        gpu = HostMesh(lib, info)
+        dev.store_mesh(gpu)
+    ref_in = [f.astype(np.longdouble) for f in host_mesh_from(lib, info, fields).fields]
+    ref_out = [f.astype(np.longdouble) for f in host_mesh_from(lib, info, prev).fields]
+    pyoracle.substep_gpu_form(ref_in, ref_out, real_params_double(info), step, 2e-3, start, end)
+    box = (slice(start[2], end[2]), slice(start[1], end[1]), slice(start[0], end[0]))
+    for q in range(8):
+        got = gpu.fields[q].astype(np.float64)
+        exp = ref_out[q].astype(np.float64)
+        outside = np.ones(got.shape, bool)
+        outside[box] = False
+        assert np.array_equal(got[outside], host_mesh_from(lib, info, prev).fields[q][outside].astype(np.float64))
+        # range-normalised: the artificial previous stage makes some outputs cancel to values far
+        # below the field's magnitude, where a pointwise ulp count says nothing
+        scale = np.max(np.abs(exp[box]))
+        tol = 8 * np.finfo(np.float64).eps if lib.double else 2e-6
+        assert np.max(np.abs(got[box] - exp[box])) / scale < tol, FIELD_NAMES[q]
 
 
 def test_fp32_error_against_fp64_oracle():

@@ -185,3 +185,48 @@ def test_error_metric_known_answers():
     # the vectorised form used by the GPU tests agrees with acGetError
     absd, ulps, rel = pyoracle.ulp_error(np.array([1.0, 3.0]), np.array([1.0 + 2.0**-52, 3.0 + 2.0**-51]), 53)
     assert np.allclose(ulps.astype(float), [1.0, 1.0])
+
+
+def test_gpu_form_rk3_pins():
+    """The two-buffer RK3 form of the reference's GPU path (integration.cuh:14-37), restated in
+    the oracle, has the same amplification factor on the Beltrami field, and agrees with the
+    host form to rounding (both against long double)."""
+    n, k, a0, lnrho, dt, steps = 16, 2, 0.1, 0.3, 0.05, 4
+    h = 2 * math.pi / n
+    f, zc = beltrami(n, k, a0, lnrho, np.longdouble)
+    pyoracle.integrate_steps_gpu_form(f, params(n), dt, steps)
+    z = -ETA * k2_discrete(k, h) * dt
+    a_n = a0 * (1 + z + z**2 / 2 + z**3 / 6) ** steps
+    sl = (slice(3, -3),) * 3
+    assert np.max(np.abs(f[4][sl].astype(np.float64) - a_n * np.sin(k * zc[sl]))) < 1e-14
+    assert np.max(np.abs(f[5][sl].astype(np.float64) - a_n * np.cos(k * zc[sl]))) < 1e-14
+
+    n = 16
+    base = mhd_fields(n, n, n, np.float64, config_id=1)
+    rp = params(n, eta=5e-3)
+    host80 = [x.astype(np.longdouble) for x in base]
+    gpu80 = [x.astype(np.longdouble) for x in base]
+    gpu64 = [x.copy() for x in base]
+    pyoracle.integrate_steps(host80, rp, 1e-3, 3)
+    pyoracle.integrate_steps_gpu_form(gpu80, rp, 1e-3, 3)
+    pyoracle.integrate_steps_gpu_form(gpu64, rp, 1e-3, 3)
+    for q in range(8):
+        scale = float(np.max(np.abs(host80[q][sl]))) + 1e-300
+        assert float(np.max(np.abs(gpu80[q][sl] - host80[q][sl]))) / scale < 1e-16
+        v = pyoracle.verify_field(host80[q][sl].astype(np.float64), gpu64[q][sl], 53, 2.2e-16)
+        assert v["ok"], (q, v)
+
+
+def test_gpu_form_substep_only_writes_region():
+    n = 10
+    f = mhd_fields(n, n, n, np.float64, config_id=8)
+    prev = [x * 0.25 for x in f]
+    out = [x.copy() for x in prev]
+    start, end = (4, 5, 3), (9, 13, 8)
+    pyoracle.substep_gpu_form(f, out, params(n, eta=5e-3), 1, 1e-3, start, end)
+    box = (slice(start[2], end[2]), slice(start[1], end[1]), slice(start[0], end[0]))
+    for q in range(8):
+        mask = np.ones(out[q].shape, bool)
+        mask[box] = False
+        assert np.array_equal(out[q][mask], prev[q][mask])
+        assert not np.array_equal(out[q][box], prev[q][box])
