@@ -68,7 +68,7 @@ struct RK3Coeffs {
 struct SubstepArgs {
     const AcReal* __restrict__ in[NF];
     AcReal* __restrict__ out[NF];
-    int mx, my, mxy;
+    int mx, my, mz, mxy;
     int3 start, end;
     PhysParams phys;
     RK3Coeffs rk;

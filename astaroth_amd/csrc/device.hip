@@ -95,6 +95,7 @@ make_args(const Device d, const AcReal* const* in, AcReal* const* out, int step,
     }
     a.mx    = nx_of(d) + 6;
     a.my    = ny_of(d) + 6;
+    a.mz    = nz_of(d) + 6;
     a.mxy   = a.mx * a.my;
     a.start = start;
     a.end   = end;

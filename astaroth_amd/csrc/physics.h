@@ -127,6 +127,136 @@ mhd_rates(const Fd<AcReal>& lr, const Fd<AcReal>& ux, const Fd<AcReal>& uy, cons
     }
 }
 
+// ---- Streaming form of the same right-hand side ----
+// The equations use each field only through a few combinations of its derivatives. Folding
+// every field's derivatives into these aggregates as soon as they are computed keeps about 37
+// values live instead of all 8 x 10 derivatives:
+//   lnrho, s : value, gradient, Laplacian
+//   u        : value, gradients, Laplacian, grad(div u)
+//   A        : Laplacian (induction), grad(div A) - lap A (current), curl A
+// The arithmetic is the same as mhd_rates up to the order of additions inside these sums.
+struct MhdAgg {
+    AcReal lr_v, lr_x, lr_y, lr_z, lr_lap;
+    AcReal s_v, s_x, s_y, s_z, s_lap;
+    AcReal uv[3], ug[3][3], ulap[3], gdu[3];
+    AcReal alap[3], jj[3], B[3];
+};
+
+// Component c (0, 1, 2) of u with its value v, first derivatives g[3] and second derivatives
+// xx, yy, zz, and the two cross derivatives it has (xy/xz for x, xy/yz for y, xz/yz for z).
+template <int C>
+__device__ __forceinline__ void
+agg_u(MhdAgg& a, AcReal v, AcReal gx, AcReal gy, AcReal gz, AcReal xx, AcReal yy, AcReal zz,
+      AcReal xy, AcReal xz, AcReal yz)
+{
+    a.uv[C]    = v;
+    a.ug[C][0] = gx;
+    a.ug[C][1] = gy;
+    a.ug[C][2] = gz;
+    a.ulap[C]  = xx + yy + zz;
+    // grad(div u)_r = sum_c H(u_c)[r][c]
+    if (C == 0) {
+        a.gdu[0] = xx;
+        a.gdu[1] = xy;
+        a.gdu[2] = xz;
+    }
+    else if (C == 1) {
+        a.gdu[0] += xy;
+        a.gdu[1] += yy;
+        a.gdu[2] += yz;
+    }
+    else {
+        a.gdu[0] += xz;
+        a.gdu[1] += yz;
+        a.gdu[2] += zz;
+    }
+}
+
+template <int C>
+__device__ __forceinline__ void
+agg_a(MhdAgg& a, AcReal gx, AcReal gy, AcReal gz, AcReal xx, AcReal yy, AcReal zz, AcReal xy,
+      AcReal xz, AcReal yz)
+{
+    const AcReal lap = xx + yy + zz;
+    a.alap[C]        = lap;
+    // jj = grad(div A) - lap A; B = curl A
+    if (C == 0) {
+        a.jj[0] = xx - lap;
+        a.jj[1] = xy;
+        a.jj[2] = xz;
+        a.B[1]  = gz;  // + dAx/dz
+        a.B[2]  = -gy; // - dAx/dy
+    }
+    else if (C == 1) {
+        a.jj[0] += xy;
+        a.jj[1] += yy - lap;
+        a.jj[2] += yz;
+        a.B[0] = -gz;    // - dAy/dz
+        a.B[2] += gx;    // + dAy/dx
+    }
+    else {
+        a.jj[0] += xz;
+        a.jj[1] += yz;
+        a.jj[2] += zz - lap;
+        a.B[0] += gy;    // + dAz/dy
+        a.B[1] -= gx;    // - dAz/dx
+    }
+}
+
+__device__ __forceinline__ void
+mhd_rates_agg(const MhdAgg& g, const PhysParams& p, AcReal rate[NF])
+{
+    typedef AcReal T;
+    const T divu = g.ug[0][0] + g.ug[1][1] + g.ug[2][2];
+    rate[VTXBUF_LNRHO] = -(g.uv[0] * g.lr_x + g.uv[1] * g.lr_y + g.uv[2] * g.lr_z) - divu;
+
+    const T S00 = T(2. / 3.) * g.ug[0][0] - T(1. / 3.) * (g.ug[1][1] + g.ug[2][2]);
+    const T S11 = T(2. / 3.) * g.ug[1][1] - T(1. / 3.) * (g.ug[0][0] + g.ug[2][2]);
+    const T S22 = T(2. / 3.) * g.ug[2][2] - T(1. / 3.) * (g.ug[0][0] + g.ug[1][1]);
+    const T S01 = T(0.5) * (g.ug[0][1] + g.ug[1][0]);
+    const T S02 = T(0.5) * (g.ug[0][2] + g.ug[2][0]);
+    const T S12 = T(0.5) * (g.ug[1][2] + g.ug[2][1]);
+
+    const T jx = p.inv_mu0 * g.jj[0], jy = p.inv_mu0 * g.jj[1], jz = p.inv_mu0 * g.jj[2];
+    const T Bx = g.B[0], By = g.B[1], Bz = g.B[2];
+
+    const T rho     = fast_exp<T>(g.lr_v);
+    const T inv_rho = fast_rcp<T>(rho);
+    const T E       = fast_exp<T>(p.gamma_inv_cp * g.s_v + p.gamma_m1 * (g.lr_v - p.lnrho0));
+    const T cs2     = p.cs2_sound * E;
+    {
+        const T jxB[3] = {jy * Bz - jz * By, jz * Bx - jx * Bz, jx * By - jy * Bx};
+        const T sg[3]  = {g.s_x, g.s_y, g.s_z};
+        const T lg[3]  = {g.lr_x, g.lr_y, g.lr_z};
+        const T Sgl[3] = {S00 * g.lr_x + S01 * g.lr_y + S02 * g.lr_z,
+                          S01 * g.lr_x + S11 * g.lr_y + S12 * g.lr_z,
+                          S02 * g.lr_x + S12 * g.lr_y + S22 * g.lr_z};
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            rate[VTXBUF_UUX + r] =
+                -(g.ug[r][0] * g.uv[0] + g.ug[r][1] * g.uv[1] + g.ug[r][2] * g.uv[2]) -
+                cs2 * (p.inv_cp * sg[r] + lg[r]) + inv_rho * jxB[r] +
+                p.nu_visc * (g.ulap[r] + T(1. / 3.) * g.gdu[r] + T(2.) * Sgl[r]) + p.zeta * g.gdu[r];
+    }
+    rate[VTXBUF_AX] = (g.uv[1] * Bz - g.uv[2] * By) + p.eta * g.alap[0];
+    rate[VTXBUF_AY] = (g.uv[2] * Bx - g.uv[0] * Bz) + p.eta * g.alap[1];
+    rate[VTXBUF_AZ] = (g.uv[0] * By - g.uv[1] * Bx) + p.eta * g.alap[2];
+    {
+        const T inv_pT   = inv_rho * fast_rcp<T>(p.exp_lnT0 * E);
+        const T contract = S00 * S00 + S11 * S11 + S22 * S22 +
+                           T(2.) * (S01 * S01 + S02 * S02 + S12 * S12);
+        const T heating = p.eta_mu0 * (jx * jx + jy * jy + jz * jz) +
+                          T(2.) * rho * p.nu_visc * contract + p.zeta * rho * divu * divu;
+        const T glT_x = p.gamma_inv_cp * g.s_x + p.gamma_m1 * g.lr_x;
+        const T glT_y = p.gamma_inv_cp * g.s_y + p.gamma_m1 * g.lr_y;
+        const T glT_z = p.gamma_inv_cp * g.s_z + p.gamma_m1 * g.lr_z;
+        const T first = p.gamma_inv_cp * g.s_lap + p.gamma_m1 * g.lr_lap;
+        const T heat  = p.k_thermal * inv_rho * (first + glT_x * glT_x + glT_y * glT_y + glT_z * glT_z);
+        rate[VTXBUF_ENTROPY] =
+            -(g.uv[0] * g.s_x + g.uv[1] * g.s_y + g.uv[2] * g.s_z) + inv_pT * heating + heat;
+    }
+}
+
 // Low-storage RK3 update of one value (integration.cuh:14-37).
 __device__ __forceinline__ AcReal
 rk3_update(const RK3Coeffs& rk, AcReal prev, AcReal cur, AcReal rate)

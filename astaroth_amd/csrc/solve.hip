@@ -355,6 +355,228 @@ solve_zmarch(const SubstepArgs a, const int zdepth, const int ntx, const int nty
 }
 
 // ------------------------------------------------------------------------------------------
+// variant 2: the z-march of variant 1 with the plane loop unrolled by the queue length
+// ------------------------------------------------------------------------------------------
+// * Vector-field staging loads are unconditional: an element outside the grid (partial tiles)
+//   reads the plane's first cell instead, and the staging buffers are padded to whole rounds of
+//   the block, so no exec-mask branch surrounds those loads or their LDS stores. Values staged
+//   outside the grid are only read by threads of partial tiles, which compute but do not store.
+// * The z loop advances 7 planes per trip with the phase PH = (p - pbeg) mod 7 known at compile
+//   time: plane p' lives in queue slot (p' - pbeg) mod 7 and output z' accumulates in slot
+//   (z' - pbeg) mod 7, so pushing a plane and retiring an output move no registers.
+template <int TX, int TY> struct ZMarch2 {
+    static constexpr int NT = TX * TY;
+    static constexpr int LX = TX + 6, LY = TY + 6, LN = LX * LY;
+    static constexpr int R  = (LN + NT - 1) / NT;
+    static constexpr int LNP = R * NT; // padded plane: every thread stores every round
+
+    struct Shared {
+        AcReal ring[4][6][LNP];
+        AcReal scal[2][LNP];
+    };
+
+    struct State {
+        AcReal q[NF][7];
+        AcReal axz[4][7];
+        AcReal ayz[4][7];
+        AcReal pv[6][R];
+        AcReal ps[2][R];
+        AcReal pc[2];
+    };
+
+    const SubstepArgs& a;
+    Shared& sh;
+    int tid, cofs, zs, ze, pend;
+    long own;
+    bool active;
+    int goff[R];
+
+    __device__ __forceinline__ ZMarch2(const SubstepArgs& args, Shared& shared) : a(args), sh(shared) {}
+
+    __device__ __forceinline__ void issue_loads(State& st, int p) const
+    {
+        const long base = (long)p * a.mxy;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            const AcReal* __restrict__ src = a.in[1 + f] + base;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                st.pv[f][r] = src[goff[r]];
+        }
+        const int zo = p - 3;
+        if (zo >= zs) {
+            const long base2 = (long)zo * a.mxy;
+            const AcReal* __restrict__ l = a.in[VTXBUF_LNRHO] + base2;
+            const AcReal* __restrict__ e = a.in[VTXBUF_ENTROPY] + base2;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                st.ps[0][r] = l[goff[r]];
+                st.ps[1][r] = e[goff[r]];
+            }
+        }
+        st.pc[0] = a.in[VTXBUF_LNRHO][base + own];
+        st.pc[1] = a.in[VTXBUF_ENTROPY][base + own];
+    }
+
+    template <int PH> __device__ __forceinline__ void step(State& st, int p) const
+    {
+        const PhysParams& P = a.phys;
+        const int slot       = p & 3;
+        const int z          = p - 3;
+        const bool out_plane = z >= zs;
+
+        __syncthreads(); // all threads are done with slot `slot` (plane p - 4) and with `scal`
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                sh.ring[slot][f][r * NT + tid] = st.pv[f][r];
+        if (out_plane) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                sh.scal[0][r * NT + tid] = st.ps[0][r];
+                sh.scal[1][r * NT + tid] = st.ps[1][r];
+            }
+        }
+        const AcReal c_lr = st.pc[0], c_ss = st.pc[1];
+        __syncthreads();
+
+        if (p + 1 < pend)
+            issue_loads(st, p + 1);
+
+        AcReal prev[NF];
+        const long oidx = own + (long)z * a.mxy;
+        const bool need_prev = out_plane && a.rk.step > 0;
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+            prev[f] = need_prev ? a.out[f][oidx] : AcReal(0);
+
+        // ---- ingest plane p ----
+        const AcReal* __restrict__ rs = &sh.ring[slot][0][0];
+        st.q[VTXBUF_LNRHO][PH]   = c_lr;
+        st.q[VTXBUF_ENTROPY][PH] = c_ss;
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+            st.q[1 + f][PH] = rs[f * LNP + cofs];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            const AcReal* __restrict__ sx = rs + XZ_FIELDS[n] * LNP;
+            const AcReal* __restrict__ sy = rs + YZ_FIELDS[n] * LNP;
+#pragma unroll
+            for (int i = 1; i <= 3; ++i) {
+                const AcReal dx = P.cxz[i] * (sx[cofs + i] - sx[cofs - i]);
+                st.axz[n][(PH - i + 7) % 7] += dx;
+                st.axz[n][(PH + i) % 7] -= dx;
+                const AcReal dy = P.cyz[i] * (sy[cofs + i * LX] - sy[cofs - i * LX]);
+                st.ayz[n][(PH - i + 7) % 7] += dy;
+                st.ayz[n][(PH + i) % 7] -= dy;
+            }
+        }
+
+        // ---- output plane z = p - 3 ----
+        constexpr int OS = (PH + 4) % 7; // accumulator slot of output p - 3
+        if (out_plane) {
+            const AcReal* __restrict__ ro = &sh.ring[z & 3][0][0];
+            Fd<AcReal> d[NF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                AcReal zq[7];
+#pragma unroll
+                for (int k = 0; k < 7; ++k)
+                    zq[k] = st.q[f][(PH + 1 + k) % 7];
+                d[f].v = zq[3];
+                axis_queue(zq, P.d1[2], P.d2[2], d[f].z, d[f].zz);
+                d[f].xy = d[f].xz = d[f].yz = 0;
+            }
+            inplane_lds<LX>(&sh.scal[0][0], cofs, P, d[VTXBUF_LNRHO].v, d[VTXBUF_LNRHO].x,
+                            d[VTXBUF_LNRHO].y, d[VTXBUF_LNRHO].xx, d[VTXBUF_LNRHO].yy);
+            inplane_lds<LX>(&sh.scal[1][0], cofs, P, d[VTXBUF_ENTROPY].v, d[VTXBUF_ENTROPY].x,
+                            d[VTXBUF_ENTROPY].y, d[VTXBUF_ENTROPY].xx, d[VTXBUF_ENTROPY].yy);
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+                inplane_lds<LX>(ro + f * LNP, cofs, P, d[1 + f].v, d[1 + f].x, d[1 + f].y,
+                                d[1 + f].xx, d[1 + f].yy);
+            d[VTXBUF_UUX].xy = cross_xy_lds<LX>(ro + R_UX * LNP, cofs, P.cxy);
+            d[VTXBUF_UUY].xy = cross_xy_lds<LX>(ro + R_UY * LNP, cofs, P.cxy);
+            d[VTXBUF_AX].xy  = cross_xy_lds<LX>(ro + R_AX * LNP, cofs, P.cxy);
+            d[VTXBUF_AY].xy  = cross_xy_lds<LX>(ro + R_AY * LNP, cofs, P.cxy);
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                d[1 + XZ_FIELDS[n]].xz = st.axz[n][OS];
+                d[1 + YZ_FIELDS[n]].yz = st.ayz[n][OS];
+            }
+            AcReal rate[NF];
+            mhd_rates(d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], P, rate);
+            if (active) {
+#pragma unroll
+                for (int f = 0; f < NF; ++f)
+                    a.out[f][oidx] = rk3_update(a.rk, prev[f], d[f].v, rate[f]);
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            st.axz[n][OS] = 0;
+            st.ayz[n][OS] = 0;
+        }
+    }
+};
+
+template <int TX, int TY>
+__global__ void __launch_bounds__(TX* TY)
+solve_zmarch2(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
+{
+    typedef ZMarch2<TX, TY> K;
+    __shared__ typename K::Shared sh;
+    K k(a, sh);
+    const int tid = threadIdx.x;
+    const int tx = tid % TX, ty = tid / TX;
+    const int b   = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int btx = b % ntx;
+    const int bty = (b / ntx) % nty;
+    const int btz = b / (ntx * nty);
+    const int x0 = a.start.x + btx * TX, y0 = a.start.y + bty * TY;
+    k.tid        = tid;
+    k.zs         = a.start.z + btz * zdepth;
+    k.ze         = min(k.zs + zdepth, a.end.z);
+    const int gx = x0 + tx, gy = y0 + ty;
+    k.active     = gx < a.end.x && gy < a.end.y;
+    k.own        = k.active ? (long)gx + (long)gy * a.mx : (long)x0 + (long)y0 * a.mx;
+#pragma unroll
+    for (int r = 0; r < K::R; ++r) {
+        const int e  = tid + r * K::NT;
+        const int ex = e % K::LX, ey = e / K::LX;
+        const int xx = x0 - 3 + ex, yy = y0 - 3 + ey;
+        k.goff[r]    = (e < K::LN && xx < a.mx && yy < a.my) ? xx + yy * a.mx : 0;
+    }
+    k.cofs = (ty + 3) * K::LX + (tx + 3);
+
+    typename K::State st;
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int j = 0; j < 7; ++j)
+            st.q[f][j] = 0;
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+        for (int j = 0; j < 7; ++j)
+            st.axz[f][j] = st.ayz[f][j] = 0;
+
+    const int pbeg = k.zs - 3;
+    k.pend         = k.ze + 3;
+    k.issue_loads(st, pbeg);
+    for (int p = pbeg; p < k.pend; p += 7) {
+        k.template step<0>(st, p);
+        if (p + 1 < k.pend) k.template step<1>(st, p + 1);
+        if (p + 2 < k.pend) k.template step<2>(st, p + 2);
+        if (p + 3 < k.pend) k.template step<3>(st, p + 3);
+        if (p + 4 < k.pend) k.template step<4>(st, p + 4);
+        if (p + 5 < k.pend) k.template step<5>(st, p + 5);
+        if (p + 6 < k.pend) k.template step<6>(st, p + 6);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // configuration table
 // ------------------------------------------------------------------------------------------
 struct ConfigEntry {
@@ -373,12 +595,22 @@ constexpr ConfigEntry kConfigs[] = {
     {1, 128, 2},
     {1, 64, 8},
 #endif
+    {2, 64, 4},
+    {2, 32, 4},
+    {2, 32, 8},
+    {2, 64, 2},
+    {2, 16, 4},
+    {2, 8, 8},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
+// Grid of z-march blocks: tiles of TX x TY columns, each marching z_depth planes (0 or more
+// than the box: the whole box depth).
+typedef void (*ZMarchKernel)(const SubstepArgs, const int, const int, const int);
+
 template <int TX, int TY>
 hipError_t
-launch_zmarch(const SubstepArgs& a, int zdepth, hipStream_t stream)
+launch_zmarch_grid(ZMarchKernel kernel, const SubstepArgs& a, int zdepth, hipStream_t stream)
 {
     const int nx = a.end.x - a.start.x, ny = a.end.y - a.start.y, nz = a.end.z - a.start.z;
     const int ntx = (nx + TX - 1) / TX, nty = (ny + TY - 1) / TY;
@@ -388,7 +620,7 @@ launch_zmarch(const SubstepArgs& a, int zdepth, hipStream_t stream)
     const long nblocks = (long)ntx * nty * ntz;
     if (nblocks > 0x7fffffffL)
         return hipErrorInvalidConfiguration;
-    solve_zmarch<TX, TY><<<dim3((unsigned)nblocks), dim3(TX * TY), 0, stream>>>(a, zdepth, ntx, nty);
+    kernel<<<dim3((unsigned)nblocks), dim3(TX * TY), 0, stream>>>(a, zdepth, ntx, nty);
     return hipGetLastError();
 }
 
@@ -456,7 +688,7 @@ launch_substep(const AcKernelConfig& c, const SubstepArgs& a, hipStream_t stream
         return launch_direct<TX, TY>(a, stream);
 #define AC_ZMARCH(TX, TY)                                                                          \
     if (c.variant == 1 && c.tile_x == TX && c.tile_y == TY)                                        \
-        return launch_zmarch<TX, TY>(a, c.z_depth, stream);
+        return launch_zmarch_grid<TX, TY>(solve_zmarch<TX, TY>, a, c.z_depth, stream);
     AC_DIRECT(64, 4)
     AC_DIRECT(32, 4)
     AC_DIRECT(64, 2)
@@ -468,8 +700,18 @@ launch_substep(const AcKernelConfig& c, const SubstepArgs& a, hipStream_t stream
     AC_ZMARCH(128, 2)
     AC_ZMARCH(64, 8)
 #endif
+#define AC_ZMARCH2(TX, TY)                                                                         \
+    if (c.variant == 2 && c.tile_x == TX && c.tile_y == TY)                                        \
+        return launch_zmarch_grid<TX, TY>(solve_zmarch2<TX, TY>, a, c.z_depth, stream);
+    AC_ZMARCH2(64, 4)
+    AC_ZMARCH2(32, 4)
+    AC_ZMARCH2(32, 8)
+    AC_ZMARCH2(64, 2)
+    AC_ZMARCH2(16, 4)
+    AC_ZMARCH2(8, 8)
 #undef AC_DIRECT
 #undef AC_ZMARCH
+#undef AC_ZMARCH2
     return hipErrorInvalidConfiguration;
 }
 
