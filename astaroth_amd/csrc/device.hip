@@ -907,7 +907,7 @@ acDeviceTimeKernelConfig(const Device device, const AcKernelConfig config, const
 }
 
 // ref device.cc:83-98 / integration.cuh:130-215. Sweeps every compiled tile shape and, for the
-// z-march variant, z-depths {8, 16, 32, 64, 128, n_z}; keeps the fastest substep. Overwrites the
+// z-march variants, z-depths below n_z plus n_z itself; keeps the fastest substep. Overwrites the
 // `out` buffers (as the reference's sweep does), so call it between full RK3 steps.
 AcResult
 acDeviceAutoOptimize(const Device device)
@@ -926,15 +926,18 @@ acDeviceAutoOptimize(const Device device)
     const int iters  = cells >= (1L << 24) ? 3 : 5;
     AcKernelConfig best = default_kernel_config(nx_of(device), ny_of(device), nz_of(device));
     float best_ms       = INFINITY;
-    const int depths[]  = {8, 16, 32, 64, 128, 0};
+    // z-depths: powers of two for variant 1, whole 7-plane rounds for the unrolled marches
+    const int depths1[] = {8, 16, 32, 64, 128, 0};
+    const int depths7[] = {14, 28, 56, 112, 0};
     for (int i = 0; i < kernel_config_count(); ++i) {
         AcKernelConfig c = kernel_config_at(i);
         if (c.tile_x > 2 * nx_of(device) || c.tile_y > 2 * ny_of(device))
             continue;
-        const int ndepth = c.variant == 1 ? (int)(sizeof(depths) / sizeof(depths[0])) : 1;
+        const int* depths = c.variant == 1 ? depths1 : depths7;
+        const int ndepth  = c.variant == 0 ? 1 : c.variant == 1 ? 6 : 5;
         for (int di = 0; di < ndepth; ++di) {
-            c.z_depth = c.variant == 1 ? depths[di] : 0;
-            if (c.z_depth > nz_of(device))
+            c.z_depth = c.variant == 0 ? 0 : depths[di];
+            if (c.z_depth >= nz_of(device))
                 continue;
             const float ms = time_config(device, c, iters);
             if (ms > 0 && ms < best_ms) {

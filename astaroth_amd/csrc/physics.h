@@ -128,80 +128,19 @@ mhd_rates(const Fd<AcReal>& lr, const Fd<AcReal>& ux, const Fd<AcReal>& uy, cons
 }
 
 // ---- Streaming form of the same right-hand side ----
-// The equations use each field only through a few combinations of its derivatives. Folding
-// every field's derivatives into these aggregates as soon as they are computed keeps about 37
-// values live instead of all 8 x 10 derivatives:
+// The equations use each field only through a few combinations of its derivatives. A kernel
+// that folds every field's derivatives into these aggregates as soon as they are formed keeps
+// about 37 values live instead of all 8 x 10 derivatives:
 //   lnrho, s : value, gradient, Laplacian
-//   u        : value, gradients, Laplacian, grad(div u)
+//   u        : value, gradients (ug[r][c] = d u_r / d x_c), Laplacian, grad(div u)
 //   A        : Laplacian (induction), grad(div A) - lap A (current), curl A
-// The arithmetic is the same as mhd_rates up to the order of additions inside these sums.
+// The arithmetic is that of mhd_rates up to the order of additions inside these sums.
 struct MhdAgg {
     AcReal lr_v, lr_x, lr_y, lr_z, lr_lap;
     AcReal s_v, s_x, s_y, s_z, s_lap;
     AcReal uv[3], ug[3][3], ulap[3], gdu[3];
     AcReal alap[3], jj[3], B[3];
 };
-
-// Component c (0, 1, 2) of u with its value v, first derivatives g[3] and second derivatives
-// xx, yy, zz, and the two cross derivatives it has (xy/xz for x, xy/yz for y, xz/yz for z).
-template <int C>
-__device__ __forceinline__ void
-agg_u(MhdAgg& a, AcReal v, AcReal gx, AcReal gy, AcReal gz, AcReal xx, AcReal yy, AcReal zz,
-      AcReal xy, AcReal xz, AcReal yz)
-{
-    a.uv[C]    = v;
-    a.ug[C][0] = gx;
-    a.ug[C][1] = gy;
-    a.ug[C][2] = gz;
-    a.ulap[C]  = xx + yy + zz;
-    // grad(div u)_r = sum_c H(u_c)[r][c]
-    if (C == 0) {
-        a.gdu[0] = xx;
-        a.gdu[1] = xy;
-        a.gdu[2] = xz;
-    }
-    else if (C == 1) {
-        a.gdu[0] += xy;
-        a.gdu[1] += yy;
-        a.gdu[2] += yz;
-    }
-    else {
-        a.gdu[0] += xz;
-        a.gdu[1] += yz;
-        a.gdu[2] += zz;
-    }
-}
-
-template <int C>
-__device__ __forceinline__ void
-agg_a(MhdAgg& a, AcReal gx, AcReal gy, AcReal gz, AcReal xx, AcReal yy, AcReal zz, AcReal xy,
-      AcReal xz, AcReal yz)
-{
-    const AcReal lap = xx + yy + zz;
-    a.alap[C]        = lap;
-    // jj = grad(div A) - lap A; B = curl A
-    if (C == 0) {
-        a.jj[0] = xx - lap;
-        a.jj[1] = xy;
-        a.jj[2] = xz;
-        a.B[1]  = gz;  // + dAx/dz
-        a.B[2]  = -gy; // - dAx/dy
-    }
-    else if (C == 1) {
-        a.jj[0] += xy;
-        a.jj[1] += yy - lap;
-        a.jj[2] += yz;
-        a.B[0] = -gz;    // - dAy/dz
-        a.B[2] += gx;    // + dAy/dx
-    }
-    else {
-        a.jj[0] += xz;
-        a.jj[1] += yz;
-        a.jj[2] += zz - lap;
-        a.B[0] += gy;    // + dAz/dy
-        a.B[1] -= gx;    // - dAz/dx
-    }
-}
 
 __device__ __forceinline__ void
 mhd_rates_agg(const MhdAgg& g, const PhysParams& p, AcReal rate[NF])

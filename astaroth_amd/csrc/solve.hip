@@ -20,6 +20,15 @@
 //      Global loads of plane p+1 are issued before plane p is processed. Blocks are remapped so
 //      that each XCD (round-robin dispatch over 8 XCDs) works on a contiguous range of tiles and
 //      neighbouring tiles share the XCD's L2.
+//
+//  variant 2: variant 1 with the plane loop unrolled by the 7-plane queue length, so queue and
+//      accumulator slots are compile-time registers and no register moves rotate them.
+//
+//  variants 3, 4 ("lean z-march", the default): all eight fields in one 4-slot LDS ring, a
+//      3-register z queue per field, six combined cross-derivative sums instead of eight, a
+//      branch-free plane loop and a right-hand side formed field by field. About 25 % fewer
+//      issued instructions per cell than variant 2; variant 4 is the same kernel compiled for
+//      two waves per SIMD.
 #include "physics.h"
 
 namespace ac {
@@ -577,6 +586,318 @@ solve_zmarch2(const SubstepArgs a, const int zdepth, const int ntx, const int nt
 }
 
 // ------------------------------------------------------------------------------------------
+// variants 3 and 4: lean z-march with all eight fields in one LDS ring
+// ------------------------------------------------------------------------------------------
+// The data flow of variant 2 with the per-thread state cut down and the plane loop free of
+// branches (variant 3 is compiled for one wave per SIMD, variant 4 for two):
+// * Every field's planes z..z+3 sit in a 4-slot LDS ring (a haloed TX x TY tile each) and only
+//   z-3..z-1 in a 3-register queue, so the z stencils read four of their seven values from LDS.
+// * The x-z and y-z cross derivatives are only used in six sums (grad div u and
+//   grad div A - lap A), so six pending accumulators per output plane are kept instead of eight:
+//     X0 = uz_xz, X1 = uz_yz, X2 = ux_xz + uy_yz, X3 = az_xz, X4 = az_yz, X5 = ax_xz + ay_yz.
+//   The first contribution to an output plane (from plane z - 3) is assigned, so no slot is
+//   ever cleared.
+// * The march is a prologue of six planes that only fill the queues, then whole rounds of 7
+//   planes that each retire one output plane. Planes past the grid are read clamped to its
+//   last plane; the outputs that depend on them (z >= ze) are not stored. Stores are the only
+//   guarded code: by z < ze (uniform) and by the thread's column lying in the box.
+// * Global loads and stores use a uniform per-plane base and a 32-bit per-thread byte offset.
+// * The right-hand side is formed field by field into the aggregates of MhdAgg, which keeps the
+//   live set small.
+template <int TX, int TY> struct ZMarchLean {
+    static constexpr int NT  = TX * TY;
+    static constexpr int LX  = TX + 6, LY = TY + 6, LN = LX * LY;
+    static constexpr int R   = (LN + NT - 1) / NT;
+    static constexpr int LNP = R * NT;
+
+    struct Shared {
+        AcReal ring[4][NF][LNP];
+    };
+
+    struct State {
+        AcReal qv[NF][3]; // planes z-3..z-1 of every field
+        AcReal x[6][7];   // cross-derivative sums, slot (z - pbeg) mod 7
+        AcReal pv[NF][R]; // staging registers of the next plane
+    };
+
+    const SubstepArgs& a;
+    Shared& sh;
+    int tid, cofs, ze, zmax;
+    unsigned own; // byte offset of the thread's column within a plane
+    bool active;
+    unsigned goff[R];
+
+    __device__ __forceinline__ ZMarchLean(const SubstepArgs& args, Shared& shared) : a(args), sh(shared) {}
+
+    static __device__ __forceinline__ unsigned pin(unsigned off)
+    {
+        asm volatile("" : "+v"(off));
+        return off;
+    }
+
+    static __device__ __forceinline__ AcReal ld(const AcReal* base, unsigned byte_off)
+    {
+        return *(const AcReal*)((const char*)base + byte_off);
+    }
+
+    __device__ __forceinline__ void issue_loads(State& st, int p) const
+    {
+        const long pl = (long)min(p, zmax) * a.mxy;
+        unsigned go[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            go[r] = pin(goff[r]);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const AcReal* src = a.in[f] + pl;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                st.pv[f][r] = ld(src, go[r]);
+        }
+    }
+
+    __device__ __forceinline__ void push_queue(State& st, const AcReal* ro) const
+    {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            st.qv[f][0] = st.qv[f][1];
+            st.qv[f][1] = st.qv[f][2];
+            st.qv[f][2] = ro[f * LNP + cofs];
+        }
+    }
+
+    // Value and z derivatives of field f at plane z: qv (z-3..z-1) and ring slots z..z+3.
+    __device__ __forceinline__ void zderivs(const State& st, int f, const AcReal* r0,
+                                            const AcReal* r1, const AcReal* r2, const AcReal* r3,
+                                            AcReal& v, AcReal& dz, AcReal& dzz) const
+    {
+        const int o        = f * LNP + cofs;
+        const AcReal zq[7] = {st.qv[f][0], st.qv[f][1], st.qv[f][2], r0[o], r1[o], r2[o], r3[o]};
+        v                  = zq[3];
+        axis_queue(zq, a.phys.d1[2], a.phys.d2[2], dz, dzz);
+    }
+
+    template <int PH, bool OUT> __device__ __forceinline__ void step(State& st, int p) const
+    {
+        const PhysParams& P = a.phys;
+        const int slot      = p & 3;
+
+        __syncthreads(); // every thread is done with ring slot `slot` (plane p - 4)
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                sh.ring[slot][f][r * NT + tid] = st.pv[f][r];
+        __syncthreads();
+        issue_loads(st, p + 1);
+
+        const int z             = p - 3;
+        const unsigned long oz  = (unsigned long)min(z, ze - 1) * a.mxy;
+        const unsigned ow       = pin(own);
+        AcReal prev[NF];
+        if (OUT && a.rk.step > 0) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                prev[f] = ld(a.out[f] + oz, ow);
+        }
+
+        // ---- ingest plane p: x-z and y-z differences into the pending sums ----
+        const AcReal* __restrict__ rs = &sh.ring[slot][0][0];
+#pragma unroll
+        for (int i = 1; i <= 3; ++i) {
+            const int lo = (PH - i + 7) % 7, hi = (PH + i) % 7; // outputs p - i and p + i
+            const AcReal cx = P.cxz[i], cy = P.cyz[i];
+            const AcReal* ux = rs + VTXBUF_UUX * LNP + cofs;
+            const AcReal* uy = rs + VTXBUF_UUY * LNP + cofs;
+            const AcReal* uz = rs + VTXBUF_UUZ * LNP + cofs;
+            const AcReal* ax = rs + VTXBUF_AX * LNP + cofs;
+            const AcReal* ay = rs + VTXBUF_AY * LNP + cofs;
+            const AcReal* az = rs + VTXBUF_AZ * LNP + cofs;
+            const AcReal d0 = cx * (uz[i] - uz[-i]);
+            const AcReal d1 = cy * (uz[i * LX] - uz[-i * LX]);
+            const AcReal d2 = cx * (ux[i] - ux[-i]) + cy * (uy[i * LX] - uy[-i * LX]);
+            const AcReal d3 = cx * (az[i] - az[-i]);
+            const AcReal d4 = cy * (az[i * LX] - az[-i * LX]);
+            const AcReal d5 = cx * (ax[i] - ax[-i]) + cy * (ay[i * LX] - ay[-i * LX]);
+            st.x[0][lo] += d0;
+            st.x[1][lo] += d1;
+            st.x[2][lo] += d2;
+            st.x[3][lo] += d3;
+            st.x[4][lo] += d4;
+            st.x[5][lo] += d5;
+            if (i == 3) { // first contribution to output p + 3
+                st.x[0][hi] = -d0;
+                st.x[1][hi] = -d1;
+                st.x[2][hi] = -d2;
+                st.x[3][hi] = -d3;
+                st.x[4][hi] = -d4;
+                st.x[5][hi] = -d5;
+            }
+            else {
+                st.x[0][hi] -= d0;
+                st.x[1][hi] -= d1;
+                st.x[2][hi] -= d2;
+                st.x[3][hi] -= d3;
+                st.x[4][hi] -= d4;
+                st.x[5][hi] -= d5;
+            }
+        }
+        const AcReal* __restrict__ ro = &sh.ring[z & 3][0][0];
+        if (!OUT) {
+            push_queue(st, ro);
+            return;
+        }
+
+        // ---- output plane z = p - 3 ----
+        constexpr int OS              = (PH + 4) % 7;
+        const AcReal* __restrict__ r1 = &sh.ring[(p + 2) & 3][0][0]; // plane z + 1
+        const AcReal* __restrict__ r2 = &sh.ring[(p + 3) & 3][0][0]; // plane z + 2
+        MhdAgg g;
+        AcReal v, dx, dy, dz, dxx, dyy, dzz;
+
+        zderivs(st, VTXBUF_LNRHO, ro, r1, r2, rs, v, dz, dzz);
+        inplane_lds<LX>(ro + VTXBUF_LNRHO * LNP, cofs, P, v, dx, dy, dxx, dyy);
+        g.lr_v = v, g.lr_x = dx, g.lr_y = dy, g.lr_z = dz, g.lr_lap = dxx + dyy + dzz;
+
+        zderivs(st, VTXBUF_ENTROPY, ro, r1, r2, rs, v, dz, dzz);
+        inplane_lds<LX>(ro + VTXBUF_ENTROPY * LNP, cofs, P, v, dx, dy, dxx, dyy);
+        g.s_v = v, g.s_x = dx, g.s_y = dy, g.s_z = dz, g.s_lap = dxx + dyy + dzz;
+
+        // grad(div u)_r = sum_c d_r d_c u_c; the x-z / y-z parts come from x[0..2]
+        zderivs(st, VTXBUF_UUX, ro, r1, r2, rs, v, dz, dzz);
+        inplane_lds<LX>(ro + VTXBUF_UUX * LNP, cofs, P, v, dx, dy, dxx, dyy);
+        const AcReal ux_xy = cross_xy_lds<LX>(ro + VTXBUF_UUX * LNP, cofs, P.cxy);
+        g.uv[0] = v, g.ug[0][0] = dx, g.ug[0][1] = dy, g.ug[0][2] = dz;
+        g.ulap[0] = dxx + dyy + dzz;
+        g.gdu[0]  = dxx + st.x[0][OS];
+        g.gdu[1]  = ux_xy + st.x[1][OS];
+
+        zderivs(st, VTXBUF_UUY, ro, r1, r2, rs, v, dz, dzz);
+        inplane_lds<LX>(ro + VTXBUF_UUY * LNP, cofs, P, v, dx, dy, dxx, dyy);
+        const AcReal uy_xy = cross_xy_lds<LX>(ro + VTXBUF_UUY * LNP, cofs, P.cxy);
+        g.uv[1] = v, g.ug[1][0] = dx, g.ug[1][1] = dy, g.ug[1][2] = dz;
+        g.ulap[1] = dxx + dyy + dzz;
+        g.gdu[0] += uy_xy;
+        g.gdu[1] += dyy;
+
+        zderivs(st, VTXBUF_UUZ, ro, r1, r2, rs, v, dz, dzz);
+        inplane_lds<LX>(ro + VTXBUF_UUZ * LNP, cofs, P, v, dx, dy, dxx, dyy);
+        g.uv[2] = v, g.ug[2][0] = dx, g.ug[2][1] = dy, g.ug[2][2] = dz;
+        g.ulap[2] = dxx + dyy + dzz;
+        g.gdu[2]  = st.x[2][OS] + dzz;
+
+        // grad(div A) - lap A; the own-axis second derivative cancels
+        zderivs(st, VTXBUF_AX, ro, r1, r2, rs, v, dz, dzz);
+        inplane_lds<LX>(ro + VTXBUF_AX * LNP, cofs, P, v, dx, dy, dxx, dyy);
+        const AcReal ax_xy = cross_xy_lds<LX>(ro + VTXBUF_AX * LNP, cofs, P.cxy);
+        g.alap[0] = dxx + dyy + dzz;
+        g.jj[0]   = st.x[3][OS] - (dyy + dzz);
+        g.jj[1]   = ax_xy + st.x[4][OS];
+        g.B[1]    = dz;
+        g.B[2]    = -dy;
+
+        zderivs(st, VTXBUF_AY, ro, r1, r2, rs, v, dz, dzz);
+        inplane_lds<LX>(ro + VTXBUF_AY * LNP, cofs, P, v, dx, dy, dxx, dyy);
+        const AcReal ay_xy = cross_xy_lds<LX>(ro + VTXBUF_AY * LNP, cofs, P.cxy);
+        g.alap[1] = dxx + dyy + dzz;
+        g.jj[0] += ay_xy;
+        g.jj[1] -= dxx + dzz;
+        g.B[0] = -dz;
+        g.B[2] += dx;
+
+        zderivs(st, VTXBUF_AZ, ro, r1, r2, rs, v, dz, dzz);
+        inplane_lds<LX>(ro + VTXBUF_AZ * LNP, cofs, P, v, dx, dy, dxx, dyy);
+        g.alap[2] = dxx + dyy + dzz;
+        g.jj[2]   = st.x[5][OS] - (dxx + dyy);
+        g.B[0] += dy;
+        g.B[1] -= dx;
+
+        AcReal rate[NF];
+        mhd_rates_agg(g, P, rate);
+        push_queue(st, ro); // qv[f][2] is now plane z
+        if (z < ze && active) {
+            if (a.rk.step == 0) {
+#pragma unroll
+                for (int f = 0; f < NF; ++f)
+                    *(AcReal*)((char*)(a.out[f] + oz) + ow) = st.qv[f][2] + a.rk.beta_dt * rate[f];
+            }
+            else {
+#pragma unroll
+                for (int f = 0; f < NF; ++f) {
+                    const AcReal cur = st.qv[f][2];
+                    *(AcReal*)((char*)(a.out[f] + oz) + ow) =
+                        cur + a.rk.beta * (a.rk.ab * (cur - prev[f]) + a.rk.dt * rate[f]);
+                }
+            }
+        }
+    }
+};
+
+template <int TX, int TY, int W>
+__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W)))
+solve_zlean(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
+{
+    typedef ZMarchLean<TX, TY> K;
+    __shared__ typename K::Shared sh;
+    K k(a, sh);
+    const int tid = threadIdx.x;
+    const int tx = tid % TX, ty = tid / TX;
+    const int b   = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int btx = b % ntx;
+    const int bty = (b / ntx) % nty;
+    const int btz = b / (ntx * nty);
+    const int x0 = a.start.x + btx * TX, y0 = a.start.y + bty * TY;
+    const int zs = a.start.z + btz * zdepth;
+    k.tid        = tid;
+    k.ze         = min(zs + zdepth, a.end.z);
+    k.zmax       = a.mz - 1;
+    const int gx = x0 + tx, gy = y0 + ty;
+    k.active     = gx < a.end.x && gy < a.end.y;
+    k.own        = (unsigned)(k.active ? gx + gy * a.mx : x0 + y0 * a.mx) * sizeof(AcReal);
+#pragma unroll
+    for (int r = 0; r < K::R; ++r) {
+        const int e  = tid + r * K::NT;
+        const int ex = e % K::LX, ey = e / K::LX;
+        const int xx = x0 - 3 + ex, yy = y0 - 3 + ey;
+        k.goff[r] = (unsigned)((e < K::LN && xx < a.mx && yy < a.my) ? xx + yy * a.mx : 0) *
+                    sizeof(AcReal);
+    }
+    k.cofs = (ty + 3) * K::LX + (tx + 3);
+
+    typename K::State st;
+    // Queue entries of planes below the march and sums of outputs below zs never reach a
+    // stored output; zero them so no undefined value enters the arithmetic.
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            st.qv[f][j] = 0;
+#pragma unroll
+    for (int n = 0; n < 6; ++n)
+#pragma unroll
+        for (int j = 0; j < 7; ++j)
+            st.x[n][j] = 0;
+
+    const int pbeg = zs - 3;
+    k.issue_loads(st, pbeg);
+    k.template step<0, false>(st, pbeg);
+    k.template step<1, false>(st, pbeg + 1);
+    k.template step<2, false>(st, pbeg + 2);
+    k.template step<3, false>(st, pbeg + 3);
+    k.template step<4, false>(st, pbeg + 4);
+    k.template step<5, false>(st, pbeg + 5);
+    for (int p = pbeg + 6; p < k.ze + 3; p += 7) {
+        k.template step<6, true>(st, p);
+        k.template step<0, true>(st, p + 1);
+        k.template step<1, true>(st, p + 2);
+        k.template step<2, true>(st, p + 3);
+        k.template step<3, true>(st, p + 4);
+        k.template step<4, true>(st, p + 5);
+        k.template step<5, true>(st, p + 6);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // configuration table
 // ------------------------------------------------------------------------------------------
 struct ConfigEntry {
@@ -601,6 +922,12 @@ constexpr ConfigEntry kConfigs[] = {
     {2, 64, 2},
     {2, 16, 4},
     {2, 8, 8},
+    {3, 16, 16},
+#if AC_DOUBLE_PRECISION == 0
+    {3, 32, 4},
+    {3, 16, 8},
+    {4, 16, 16},
+#endif
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
@@ -668,13 +995,28 @@ kernel_config_at(int index)
 AcKernelConfig
 default_kernel_config(int nx, int ny, int nz)
 {
-    (void)nx;
-    (void)ny;
+    // The lean z-march (variant 3) measured fastest at 512^3: 32 x 4 tiles in FP32 (two blocks
+    // of 2 waves per CU), 16 x 16 in FP64 (one block of 4 waves per CU; its 128 KiB LDS ring
+    // leaves no room for a second). The z extent of a block is cut so that the grid still has
+    // about four blocks per concurrently resident block slot, in multiples of the 7-plane round.
     AcKernelConfig c;
-    c.variant = 1;
-    c.tile_x  = 64;
-    c.tile_y  = 4;
-    c.z_depth = nz < 64 ? nz : 64;
+    c.variant = 3;
+#if AC_DOUBLE_PRECISION == 0
+    c.tile_x = 32;
+    c.tile_y = 4;
+    const long want = 2048;
+#else
+    c.tile_x = 16;
+    c.tile_y = 16;
+    const long want = 1024;
+#endif
+    const long cols = (long)((nx + c.tile_x - 1) / c.tile_x) * ((ny + c.tile_y - 1) / c.tile_y);
+    const long nzb  = cols >= want ? 1 : (want + cols - 1) / cols;
+    int zd          = (int)((nz + nzb - 1) / nzb);
+    zd              = ((zd + 6) / 7) * 7;
+    if (zd < 14)
+        zd = 14;
+    c.z_depth = zd >= nz ? 0 : zd;
     return c;
 }
 
@@ -709,6 +1051,16 @@ launch_substep(const AcKernelConfig& c, const SubstepArgs& a, hipStream_t stream
     AC_ZMARCH2(64, 2)
     AC_ZMARCH2(16, 4)
     AC_ZMARCH2(8, 8)
+    if (c.variant == 3 && c.tile_x == 16 && c.tile_y == 16)
+        return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 1>, a, c.z_depth, stream);
+#if AC_DOUBLE_PRECISION == 0
+    if (c.variant == 3 && c.tile_x == 32 && c.tile_y == 4)
+        return launch_zmarch_grid<32, 4>(solve_zlean<32, 4, 1>, a, c.z_depth, stream);
+    if (c.variant == 3 && c.tile_x == 16 && c.tile_y == 8)
+        return launch_zmarch_grid<16, 8>(solve_zlean<16, 8, 1>, a, c.z_depth, stream);
+    if (c.variant == 4 && c.tile_x == 16 && c.tile_y == 16)
+        return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2>, a, c.z_depth, stream);
+#endif
 #undef AC_DIRECT
 #undef AC_ZMARCH
 #undef AC_ZMARCH2
