@@ -122,6 +122,12 @@ class Library:
             "acDeviceRandomizeUniform": (i, [vp, i, i, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong]),
             "acDeviceGetStreamHandle": (vp, [vp, i]),
             "acDeviceCopyVertexBuffer": (i, [vp, i, i]),
+            "acDevicePackData": (i, [vp, i, I3, I3, vp]),
+            "acDeviceUnpackData": (i, [vp, i, vp, I3, I3]),
+            "acDeviceHaloPackedSize": (ctypes.c_size_t, [vp]),
+            "acDevicePackHalo": (i, [vp, i, vp]),
+            "acDeviceUnpackHalo": (i, [vp, i, vp]),
+            "acDeviceHaloExchangeSelf": (i, [vp, i]),
             "acHostUpdateBuiltinParams": (i, [ctypes.POINTER(MI)]),
             "acHostMeshCreate": (i, [MI, ctypes.POINTER(M)]),
             "acHostMeshRandomize": (i, [ctypes.POINTER(M)]),
@@ -367,6 +373,32 @@ class Device:
 
     def stream_handle(self, stream=abi.STREAM_DEFAULT):
         return self.lib.dll.acDeviceGetStreamHandle(self.handle, stream)
+
+    # ---- halo pack / unpack; `ptr` is a device address (int) of a caller-owned buffer ----
+    def pack_data(self, start, dims, ptr, stream=abi.STREAM_DEFAULT):
+        return self._check(self.lib.dll.acDevicePackData(self.handle, stream, abi.int3(*start),
+                                                         abi.int3(*dims), ctypes.c_void_p(ptr)),
+                           "acDevicePackData")
+
+    def unpack_data(self, ptr, start, dims, stream=abi.STREAM_DEFAULT):
+        return self._check(self.lib.dll.acDeviceUnpackData(self.handle, stream, ctypes.c_void_p(ptr),
+                                                           abi.int3(*start), abi.int3(*dims)),
+                           "acDeviceUnpackData")
+
+    def halo_packed_size(self) -> int:
+        return int(self.lib.dll.acDeviceHaloPackedSize(self.handle))
+
+    def pack_halo(self, ptr, stream=abi.STREAM_DEFAULT):
+        return self._check(self.lib.dll.acDevicePackHalo(self.handle, stream, ctypes.c_void_p(ptr)),
+                           "acDevicePackHalo")
+
+    def unpack_halo(self, ptr, stream=abi.STREAM_DEFAULT):
+        return self._check(self.lib.dll.acDeviceUnpackHalo(self.handle, stream, ctypes.c_void_p(ptr)),
+                           "acDeviceUnpackHalo")
+
+    def halo_exchange_self(self, stream=abi.STREAM_DEFAULT):
+        return self._check(self.lib.dll.acDeviceHaloExchangeSelf(self.handle, stream),
+                           "acDeviceHaloExchangeSelf")
 
 
 def _rt(r):

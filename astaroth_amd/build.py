@@ -23,7 +23,7 @@ LIBDIR = os.path.join(PKG, "lib")
 BUILD = os.path.join(REPO, "_build")
 INCLUDE = os.path.join(REPO, "include")
 
-HIP_SOURCES = ["device.hip", "solve.hip", "boundconds.hip", "reductions.hip", "init.hip"]
+HIP_SOURCES = ["device.hip", "solve.hip", "boundconds.hip", "reductions.hip", "init.hip", "halo.hip"]
 HOST_SOURCES = ["host.cc"]
 ARCH = "gfx950"
 

@@ -101,6 +101,13 @@ int reduce_partials_needed();
 hipError_t launch_fill_uniform(AcReal* field, const AcMeshInfo& info, double lo, double hi,
                                unsigned long long seed, hipStream_t stream);
 hipError_t launch_fill_constant(AcReal* field, long n, AcReal value, hipStream_t stream);
+// Halo pack / unpack (halo.hip): moves the eight fields of `nblocks` blocks (origins, dims)
+// to (pack) or from (unpack) a buffer laid out block after block, each [f][z][y][x].
+hipError_t launch_pack(bool pack, AcReal* const* fields, int mx, int my, AcReal* packed,
+                       int nblocks, const int3* origins, const int3* dims, hipStream_t stream);
+// The 26 ghost blocks of an n-cell box (8 corners, 12 edges, 6 sides) with their periodic
+// interior images; returns the block count.
+int halo_blocks(int3 n, int3* ghost_origin, int3* interior_origin, int3* dims);
 // Plain element copy (one read and one write per element).
 hipError_t launch_copy(AcReal* dst, const AcReal* src, long n, hipStream_t stream);
 
@@ -120,5 +127,7 @@ struct device_s {
     // the graph was recorded for)
     std::map<std::pair<int, unsigned long long>, hipGraphExec_t> graphs[2];
     int parity; // number of swaps mod 2 since creation
+    AcReal* halo_buffer;     // acDeviceHaloExchangeSelf's pack buffer, allocated on first use
+    size_t halo_buffer_len;  // in AcReal elements
     AcReal* base_in[ac::NF];
 };

@@ -446,6 +446,8 @@ acDeviceCreate(const int id, const AcMeshInfo config, Device* device_handle)
     d->id           = id;
     d->local_config = config;
     d->parity       = 0;
+    d->halo_buffer     = nullptr;
+    d->halo_buffer_len = 0;
     for (int i = 0; i < NUM_STREAMS; ++i)
         AC_HIP_CHECK(hipStreamCreateWithFlags(&d->streams[i], hipStreamNonBlocking));
 
@@ -487,6 +489,8 @@ acDeviceDestroy(Device device)
     AC_HIP_CHECK(hipFree(device->reduce_partials));
     AC_HIP_CHECK(hipFree(device->reduce_result));
     AC_HIP_CHECK(hipHostFree(device->reduce_result_host));
+    if (device->halo_buffer)
+        AC_HIP_CHECK(hipFree(device->halo_buffer));
     for (int i = 0; i < NUM_STREAMS; ++i)
         AC_HIP_CHECK(hipStreamDestroy(device->streams[i]));
     delete device;
@@ -710,6 +714,121 @@ acDevicePeriodicBoundconds(const Device device, const Stream stream, const int3 
     AC_HIP_CHECK(launch_periodic_fill(device->vba.in, NF, nx_of(device), ny_of(device),
                                       nz_of(device), start, end, device->streams[stream]));
     return AC_SUCCESS;
+}
+
+// ---- halo pack / unpack (packing.cuh:3-92; Grid-layer blocks device.cc:1375-1381,1758-1818) ----
+
+static bool
+block_ok(const Device d, int3 start, int3 dims)
+{
+    const int m[3] = {nx_of(d) + 2 * NGHOST, ny_of(d) + 2 * NGHOST, nz_of(d) + 2 * NGHOST};
+    const int s[3] = {start.x, start.y, start.z};
+    const int e[3] = {dims.x, dims.y, dims.z};
+    for (int a = 0; a < 3; ++a)
+        if (s[a] < 0 || e[a] < 0 || s[a] + e[a] > m[a])
+            return false;
+    return true;
+}
+
+AcResult
+acDevicePackData(const Device device, const Stream stream, const int3 vba_start, const int3 dims,
+                 AcReal* packed)
+{
+    if (!stream_ok(device, stream) || !block_ok(device, vba_start, dims) || !packed)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(device->id));
+    AC_HIP_CHECK(launch_pack(true, device->vba.in, nx_of(device) + 2 * NGHOST,
+                             ny_of(device) + 2 * NGHOST, packed, 1, &vba_start, &dims,
+                             device->streams[stream]));
+    return AC_SUCCESS;
+}
+
+AcResult
+acDeviceUnpackData(const Device device, const Stream stream, const AcReal* packed,
+                   const int3 vba_start, const int3 dims)
+{
+    if (!stream_ok(device, stream) || !block_ok(device, vba_start, dims) || !packed)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(device->id));
+    AC_HIP_CHECK(launch_pack(false, device->vba.in, nx_of(device) + 2 * NGHOST,
+                             ny_of(device) + 2 * NGHOST, const_cast<AcReal*>(packed), 1,
+                             &vba_start, &dims, device->streams[stream]));
+    return AC_SUCCESS;
+}
+
+// The 26 blocks need n >= NGHOST on every axis so that each ghost block's periodic image is one
+// interior block (as in the reference Grid layer); smaller boxes get 0 blocks and AC_FAILURE.
+static int
+device_halo_blocks(const Device d, int3* ghost, int3* interior, int3* dims)
+{
+    if (nx_of(d) < NGHOST || ny_of(d) < NGHOST || nz_of(d) < NGHOST)
+        return 0;
+    return halo_blocks(make_int3(nx_of(d), ny_of(d), nz_of(d)), ghost, interior, dims);
+}
+
+size_t
+acDeviceHaloPackedSize(const Device device)
+{
+    int3 g[26], in[26], dims[26];
+    const int nb = device_halo_blocks(device, g, in, dims);
+    size_t cells = 0;
+    for (int i = 0; i < nb; ++i)
+        cells += (size_t)dims[i].x * dims[i].y * dims[i].z;
+    return cells * NF;
+}
+
+AcResult
+acDevicePackHalo(const Device device, const Stream stream, AcReal* packed)
+{
+    if (!stream_ok(device, stream) || !packed)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(device->id));
+    int3 g[26], in[26], dims[26];
+    const int nb = device_halo_blocks(device, g, in, dims);
+    if (nb == 0)
+        return AC_FAILURE;
+    AC_HIP_CHECK(launch_pack(true, device->vba.in, nx_of(device) + 2 * NGHOST,
+                             ny_of(device) + 2 * NGHOST, packed, nb, in, dims,
+                             device->streams[stream]));
+    return AC_SUCCESS;
+}
+
+AcResult
+acDeviceUnpackHalo(const Device device, const Stream stream, const AcReal* packed)
+{
+    if (!stream_ok(device, stream) || !packed)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(device->id));
+    int3 g[26], in[26], dims[26];
+    const int nb = device_halo_blocks(device, g, in, dims);
+    if (nb == 0)
+        return AC_FAILURE;
+    AC_HIP_CHECK(launch_pack(false, device->vba.in, nx_of(device) + 2 * NGHOST,
+                             ny_of(device) + 2 * NGHOST, const_cast<AcReal*>(packed), nb, g,
+                             dims, device->streams[stream]));
+    return AC_SUCCESS;
+}
+
+AcResult
+acDeviceHaloExchangeSelf(const Device device, const Stream stream)
+{
+    if (!stream_ok(device, stream))
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(device->id));
+    const size_t len = acDeviceHaloPackedSize(device);
+    if (len == 0)
+        return AC_FAILURE;
+    if (device->halo_buffer_len < len) {
+        if (device->halo_buffer) {
+            AC_HIP_CHECK(hipStreamSynchronize(device->streams[stream]));
+            AC_HIP_CHECK(hipFree(device->halo_buffer));
+        }
+        AC_HIP_CHECK(hipMalloc((void**)&device->halo_buffer, len * sizeof(AcReal)));
+        device->halo_buffer_len = len;
+    }
+    if (acDevicePackHalo(device, stream, device->halo_buffer) != AC_SUCCESS)
+        return AC_FAILURE;
+    return acDeviceUnpackHalo(device, stream, device->halo_buffer);
 }
 
 AcResult

@@ -32,34 +32,7 @@ VALU_FP32_PEAK_TFLOPS = 157.3  # vendor figure (counts packed FP32; packed math 
 VALU_FP64_PEAK_TFLOPS = 78.6
 
 
-class Hip:
-    def __init__(self):
-        self.dll = ctypes.CDLL("libamdhip64.so")
-        self.dll.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
-        self.dll.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
-        self.dll.hipEventSynchronize.argtypes = [ctypes.c_void_p]
-        self.dll.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
-        self.dll.hipEventDestroy.argtypes = [ctypes.c_void_p]
-
-    def event(self):
-        ev = ctypes.c_void_p()
-        if self.dll.hipEventCreate(ctypes.byref(ev)) != 0:
-            raise RuntimeError("hipEventCreate failed")
-        return ev
-
-    def record(self, ev, stream):
-        if self.dll.hipEventRecord(ev, stream) != 0:
-            raise RuntimeError("hipEventRecord failed")
-
-    def elapsed(self, a, b):
-        ms = ctypes.c_float()
-        self.dll.hipEventSynchronize(b)
-        if self.dll.hipEventElapsedTime(ctypes.byref(ms), a, b) != 0:
-            raise RuntimeError("hipEventElapsedTime failed")
-        return ms.value
-
-    def sync(self):
-        self.dll.hipDeviceSynchronize()
+from astaroth_amd.hiprt import Hip  # noqa: E402  (events on the library's own HIP runtime)
 
 
 def run_gpu(double: bool, n: int, steps: int, warmup: int, dt: float, autotune: bool, hip: Hip,

@@ -38,7 +38,10 @@ AcResult acDeviceIntegrateStepGraph(const Device device, const Stream stream, co
 /* ---- Solve-kernel configuration and autotuning ----
  * variant 0: one thread per cell, stencils read straight from global memory.
  * variant 1: x-y tile in LDS with a 3-cell halo, z-march of depth z_depth, register queues for
- *            the z stencils and accumulators for the x-z / y-z cross derivatives. */
+ *            the z stencils and accumulators for the x-z / y-z cross derivatives.
+ * variant 2: variant 1 with the plane loop unrolled by the 7-plane queue length.
+ * variants 3, 4: the lean z-march (all eight fields in one LDS ring, 3-register z queues, six
+ *            combined cross-derivative sums), compiled for one and two waves per SIMD. */
 typedef struct {
     int variant;
     int tile_x;
@@ -68,6 +71,32 @@ AcResult acDeviceRandomizeUniform(const Device device, const Stream stream,
  * One read and one write per cell: the known-size kernel used to calibrate memory counters. */
 AcResult acDeviceCopyVertexBuffer(const Device device, const Stream stream,
                                   const VertexBufferHandle handle);
+
+/* ---- Halo pack / unpack ----
+ * Device-level forms of the reference's acKernelPackData / acKernelUnpackData
+ * (src/core/kernels/kernels.h:65-70, packing.cuh:64-92): copy the block of `dims` cells at
+ * `vba_start` (local m-coordinates) of the eight `in` fields to / from `packed`, a device buffer
+ * of 8 * dims.x * dims.y * dims.z AcReal laid out [field][z][y][x]. Asynchronous on `stream`.
+ * A block that does not lie inside the m^3 box is rejected with AC_FAILURE. */
+AcResult acDevicePackData(const Device device, const Stream stream, const int3 vba_start,
+                          const int3 dims, AcReal* packed);
+AcResult acDeviceUnpackData(const Device device, const Stream stream, const AcReal* packed,
+                            const int3 vba_start, const int3 dims);
+
+/* The 26 halo blocks of the reference's Grid layer (src/core/device.cc:1375-1381 extents,
+ * 1758-1818 ghost-side origins; corners included) packed in one launch: 8 corners, 4 x-edges,
+ * 4 y-edges, 4 z-edges, 2 xy-sides, 2 xz-sides, 2 yz-sides, each block [field][z][y][x], blocks
+ * back to back. acDevicePackHalo reads each block's periodic interior image
+ * (a0 = ((b0 - 3) mod n) + 3, device.cc:1102-1116); acDeviceUnpackHalo writes the ghost blocks
+ * b0. acDeviceHaloPackedSize is the buffer length in AcReal. Boxes with fewer than 3 interior
+ * cells on an axis have no such blocks: size 0 and AC_FAILURE. */
+size_t acDeviceHaloPackedSize(const Device device);
+AcResult acDevicePackHalo(const Device device, const Stream stream, AcReal* packed);
+AcResult acDeviceUnpackHalo(const Device device, const Stream stream, const AcReal* packed);
+/* Pack followed by unpack through a device-owned buffer (allocated on the first call): the
+ * halo exchange of a box that is its own periodic neighbour on every side. Fills every ghost
+ * cell exactly as acDevicePeriodicBoundconds over the whole m^3 box does. */
+AcResult acDeviceHaloExchangeSelf(const Device device, const Stream stream);
 
 /* ---- Stream access for timing from the caller (returns the hipStream_t as void*) ---- */
 void* acDeviceGetStreamHandle(const Device device, const Stream stream);
