@@ -122,6 +122,8 @@ class Library:
             "acDeviceRandomizeUniform": (i, [vp, i, i, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong]),
             "acDeviceGetStreamHandle": (vp, [vp, i]),
             "acDeviceCopyVertexBuffer": (i, [vp, i, i]),
+            "acDeviceRandomizeUniformSinZ": (i, [vp, i, i, ctypes.c_double, ctypes.c_double,
+                                                 ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double]),
             "acDevicePackData": (i, [vp, i, I3, I3, vp]),
             "acDeviceUnpackData": (i, [vp, i, vp, I3, I3]),
             "acDeviceHaloPackedSize": (ctypes.c_size_t, [vp]),
@@ -351,6 +353,11 @@ class Device:
     def randomize_uniform(self, handle, lo, hi, seed, stream=abi.STREAM_DEFAULT):
         return self._check(self.lib.dll.acDeviceRandomizeUniform(self.handle, stream, _vh(handle), lo, hi, seed),
                            "acDeviceRandomizeUniform")
+
+    def randomize_uniform_sinz(self, handle, lo, hi, seed, amplitude, phase_per_cell, stream=abi.STREAM_DEFAULT):
+        return self._check(self.lib.dll.acDeviceRandomizeUniformSinZ(self.handle, stream, _vh(handle), lo, hi,
+                                                                     seed, amplitude, phase_per_cell),
+                           "acDeviceRandomizeUniformSinZ")
 
     def kernel_config(self) -> abi.KernelConfig:
         c = abi.KernelConfig()

@@ -96,10 +96,12 @@ hipError_t launch_reduce(int rtype, int nvals, const AcReal* const* vals, const 
                          hipStream_t stream);
 int reduce_partials_needed();
 
-// Device-side PRNG fill (init.hip): value = lo + (hi - lo) * u, u = splitmix64(seed, cell)
-// as a double in [0, 1), over the interior [n_min, n_max) of one buffer.
+// Device-side PRNG fill (init.hip): value = lo + (hi - lo) * u (+ amp sin(kdz (k - 3))),
+// u = splitmix64(seed, cell) as a double in [0, 1), over the interior [n_min, n_max) of one
+// buffer.
 hipError_t launch_fill_uniform(AcReal* field, const AcMeshInfo& info, double lo, double hi,
-                               unsigned long long seed, hipStream_t stream);
+                               unsigned long long seed, hipStream_t stream, double amp = 0.0,
+                               double kdz = 0.0);
 hipError_t launch_fill_constant(AcReal* field, long n, AcReal value, hipStream_t stream);
 // Halo pack / unpack (halo.hip): moves the eight fields of `nblocks` blocks (origins, dims)
 // to (pack) or from (unpack) a buffer laid out block after block, each [f][z][y][x].

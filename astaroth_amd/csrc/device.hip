@@ -1086,6 +1086,21 @@ acDeviceRandomizeUniform(const Device device, const Stream stream, const VertexB
 }
 
 AcResult
+acDeviceRandomizeUniformSinZ(const Device device, const Stream stream,
+                             const VertexBufferHandle handle, const double lo, const double hi,
+                             const unsigned long long seed, const double amplitude,
+                             const double phase_per_cell)
+{
+    if (!stream_ok(device, stream) || handle < 0 || handle >= NF || !std::isfinite(amplitude) ||
+        !std::isfinite(phase_per_cell))
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(device->id));
+    AC_HIP_CHECK(launch_fill_uniform(device->vba.in[handle], device->local_config, lo, hi, seed,
+                                     device->streams[stream], amplitude, phase_per_cell));
+    return AC_SUCCESS;
+}
+
+AcResult
 acDeviceCopyVertexBuffer(const Device device, const Stream stream, const VertexBufferHandle handle)
 {
     if (!stream_ok(device, stream) || handle < 0 || handle >= NF)

@@ -2,7 +2,8 @@
 //
 // The generator is splitmix64 applied to (seed + (cell + 1) * golden gamma), where `cell` is the
 // linear interior index (i - 3) + (j - 3) n_x + (k - 3) n_x n_y; the top 53 bits give
-// u in [0, 1) and the value is lo + (hi - lo) u evaluated in FP64 and rounded to AcReal.
+// u in [0, 1) and the value is lo + (hi - lo) u (+ amp sin(kdz (k - 3)) when amp != 0)
+// evaluated in FP64 and rounded once to AcReal.
 // Being counter based it needs no state, gives the same numbers for any launch shape, and is
 // reproduced exactly by astaroth_amd/initial.py on the host.
 #include "ac_internal.h"
@@ -20,10 +21,10 @@ splitmix64(unsigned long long z)
 }
 
 __global__ void __launch_bounds__(256)
-fill_uniform_kernel(AcReal* f, int nx, int ny, int nz, int mx, int mxy, double lo, double hi,
-                    unsigned long long seed)
+fill_uniform_kernel(AcReal* f, int nx, int ny, int nz, int mx, long mxy, double lo, double hi,
+                    unsigned long long seed, double amp, double kdz)
 {
-#pragma clang fp contract(off) // lo + (hi - lo) u rounds exactly as on the host
+#pragma clang fp contract(off) // lo + (hi - lo) u (+ amp sin) rounds exactly as on the host
     const long t     = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long total = (long)nx * ny * nz;
     if (t >= total)
@@ -34,7 +35,10 @@ fill_uniform_kernel(AcReal* f, int nx, int ny, int nz, int mx, int mxy, double l
     const int k = (int)(r / ny);
     const unsigned long long z = splitmix64(seed + (unsigned long long)(t + 1) * 0x9E3779B97F4A7C15ULL);
     const double u             = (double)(z >> 11) * (1.0 / 9007199254740992.0);
-    f[(i + 3) + (long)(j + 3) * mx + (long)(k + 3) * mxy] = (AcReal)(lo + (hi - lo) * u);
+    double v                   = lo + (hi - lo) * u;
+    if (amp != 0.0)
+        v = v + amp * sin(kdz * (double)k);
+    f[(i + 3) + (long)(j + 3) * mx + (long)(k + 3) * mxy] = (AcReal)v;
 }
 
 __global__ void __launch_bounds__(256)
@@ -67,7 +71,7 @@ launch_copy(AcReal* dst, const AcReal* src, long n, hipStream_t stream)
 
 hipError_t
 launch_fill_uniform(AcReal* field, const AcMeshInfo& info, double lo, double hi,
-                    unsigned long long seed, hipStream_t stream)
+                    unsigned long long seed, hipStream_t stream, double amp, double kdz)
 {
     const int nx = info.int_params[AC_nx], ny = info.int_params[AC_ny], nz = info.int_params[AC_nz];
     const int mx = nx + 6, my = ny + 6;
@@ -75,8 +79,8 @@ launch_fill_uniform(AcReal* field, const AcMeshInfo& info, double lo, double hi,
     const long blocks = (total + 255) / 256;
     if (blocks <= 0)
         return hipSuccess;
-    fill_uniform_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(field, nx, ny, nz, mx,
-                                                                          mx * my, lo, hi, seed);
+    fill_uniform_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(
+        field, nx, ny, nz, mx, (long)mx * my, lo, hi, seed, amp, kdz);
     return hipGetLastError();
 }
 

@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .abi import VTX
+from .abi import IP, VTX
 
 GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 M1 = np.uint64(0xBF58476D1CE4E5B9)
@@ -68,11 +68,16 @@ def mhd_fields(nx, ny, nz, dtype=np.float64, config_id=1, box_id=0, perturb_sin2
     return out
 
 
-def device_init_mhd(dev, config_id=1, box_id=0):
-    """The same distribution generated on the GPU (acDeviceRandomizeUniform + periodic fill)."""
+def device_init_mhd(dev, config_id=1, box_id=0, perturb_sin2z=False):
+    """The same distribution generated on the GPU (acDeviceRandomizeUniform + periodic fill);
+    with perturb_sin2z, u_x also gets + 1e-2 sin(2 z) (acDeviceRandomizeUniformSinZ)."""
     for name, (lo, hi) in FIELD_RANGES.items():
         q = VTX[name]
-        if hi == lo:
+        if perturb_sin2z and name == "VTXBUF_UUX":
+            nz = dev.info.int_params[IP["AC_nz"]]
+            dsz = 2 * np.pi / nz  # the grid spacing of mhd_fields, in FP64
+            dev.randomize_uniform_sinz(q, lo, hi, seed_for(config_id, box_id, q), 1e-2, 2 * dsz)
+        elif hi == lo:
             dev.set_vertex_buffer(q, lo)
         else:
             dev.randomize_uniform(q, lo, hi, seed_for(config_id, box_id, q))

@@ -36,7 +36,7 @@ from astaroth_amd.hiprt import Hip  # noqa: E402  (events on the library's own H
 
 
 def run_gpu(double: bool, n: int, steps: int, warmup: int, dt: float, autotune: bool, hip: Hip,
-            kernel_config=None):
+            kernel_config=None, halo="fill", reduce_each_step=False, perturb=False):
     from astaroth_amd.api import Library
     from astaroth_amd.initial import device_init_mhd
 
@@ -48,13 +48,14 @@ def run_gpu(double: bool, n: int, steps: int, warmup: int, dt: float, autotune: 
     try:
         if kernel_config is not None:
             dev.set_kernel_config(*kernel_config)
-        device_init_mhd(dev, config_id=1)
+        device_init_mhd(dev, config_id=3 if perturb else 1, perturb_sin2z=perturb)
         stream = dev.stream_handle()
         nmin, nmax = (3, 3, 3), (n + 3, n + 3, n + 3)
         for _ in range(warmup):
             dev.integrate_step(dt)
         dev.synchronize()
         evs = [[hip.event() for _ in range(6)] for _ in range(steps)]
+        hev = [[hip.event() for _ in range(6)] for _ in range(steps)]
         ev_begin, ev_end = hip.event(), hip.event()
         d = lib.dll
         from astaroth_amd.abi import int3
@@ -64,19 +65,28 @@ def run_gpu(double: bool, n: int, steps: int, warmup: int, dt: float, autotune: 
         hip.sync()
         wall0 = time.perf_counter()
         hip.record(ev_begin, stream)
+        reductions = []
         for k in range(steps):
             e = evs[k]
             for s in range(3):
-                d.acDevicePeriodicBoundconds(dev.handle, 0, zero, m)
+                hip.record(hev[k][2 * s], stream)
+                if halo == "fill":
+                    d.acDevicePeriodicBoundconds(dev.handle, 0, zero, m)
+                else:  # 26-block pack + unpack through a device buffer
+                    d.acDeviceHaloExchangeSelf(dev.handle, 0)
+                hip.record(hev[k][2 * s + 1], stream)
                 hip.record(e[2 * s], stream)
                 d.acDeviceIntegrateSubstep(dev.handle, 0, s, s_min, s_max, lib.real(dt))
                 hip.record(e[2 * s + 1], stream)
                 d.acDeviceSwapBuffers(dev.handle)
+            if reduce_each_step:  # FP64 diagnostics (synchronous, as the reference's reductions)
+                reductions.append(step_diagnostics(dev))
         hip.record(ev_end, stream)
         hip.sync()
         wall = time.perf_counter() - wall0
         total_ms = hip.elapsed(ev_begin, ev_end)
         solve_ms = [hip.elapsed(e[2 * s], e[2 * s + 1]) for e in evs for s in range(3)]
+        halo_ms = [hip.elapsed(e[2 * s], e[2 * s + 1]) for e in hev for s in range(3)]
         step_ms = []
         prev = ev_begin
         for e in evs:
@@ -108,7 +118,81 @@ def run_gpu(double: bool, n: int, steps: int, warmup: int, dt: float, autotune: 
         wall_s=wall,
         autotune_s=tune_s,
         kernel_config=dict(variant=cfg.variant, tile_x=cfg.tile_x, tile_y=cfg.tile_y, z_depth=cfg.z_depth),
+        reductions=reductions,
+        halo_ms_mean=sum(halo_ms) / len(halo_ms),
+        # ghost shell read + written once per field (fill), or read, packed, unpacked (pack)
+        halo_alg_bytes=float((n + 6) ** 3 - n ** 3) * 8 * s * (2 if halo == "fill" else 4),
     )
+
+
+def step_diagnostics(dev):
+    """FP64 max/min/rms of |u|, |A| and lnrho over the interior (RMS finalised as sqrt(sum/N),
+    as the reference's node layer does, node.cc:932-957)."""
+    n3 = float(dev.info.int_params[abi_ip("AC_nxyz")])
+    out = {}
+    for name, args in (("uu", ("VTXBUF_UUX", "VTXBUF_UUY", "VTXBUF_UUZ")),
+                       ("aa", ("VTXBUF_AX", "VTXBUF_AY", "VTXBUF_AZ"))):
+        out[name] = [dev.reduce_vec(r, *args, double=True) for r in ("RTYPE_MAX", "RTYPE_MIN")]
+        out[name].append((dev.reduce_vec("RTYPE_RMS", *args, double=True) / n3) ** 0.5)
+    out["lnrho"] = [dev.reduce_scal(r, "VTXBUF_LNRHO", double=True) for r in ("RTYPE_MAX", "RTYPE_MIN")]
+    out["lnrho"].append((dev.reduce_scal("RTYPE_RMS", "VTXBUF_LNRHO", double=True) / n3) ** 0.5)
+    return out
+
+
+def abi_ip(name):
+    from astaroth_amd.abi import IP
+    return IP[name]
+
+
+def precision_study(n: int, steps: int, dt: float, hip: Hip):
+    """512^3 precision study: the same perturbed config-1 state (u_x += 1e-2 sin 2z) advanced
+    `steps` RK3 steps by the FP32 and the FP64 library; per-field error of FP32 against FP64 over
+    the interior, range-normalised and at the point of maximum absolute error."""
+    import numpy as np
+    from astaroth_amd.api import Library, HostMesh
+    from astaroth_amd.initial import device_init_mhd
+    from astaroth_amd.abi import VTXBUF_NAMES
+
+    states, diag, times = {}, {}, {}
+    for prec in ("fp64", "fp32"):
+        lib = Library(prec == "fp64")
+        info = lib.config(n=n)
+        dev = lib.device_create(0, info, autotune=False)
+        try:
+            device_init_mhd(dev, config_id=3, perturb_sin2z=True)
+            diag[prec] = []
+            hip.sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                dev.integrate_step(dt)
+                diag[prec].append(step_diagnostics(dev))
+            hip.sync()
+            times[prec] = (time.perf_counter() - t0) / steps
+            mesh = HostMesh(lib, info)
+            dev.store_mesh(mesh)
+            dev.synchronize()
+            states[prec] = [f[3:-3, 3:-3, 3:-3] for f in mesh.fields]
+        finally:
+            dev.destroy()
+    fields = {}
+    worst = 0.0
+    for q, name in enumerate(VTXBUF_NAMES):
+        ref = states["fp64"][q]
+        d = np.abs(states["fp32"][q].astype(np.float64) - ref)
+        i = int(np.argmax(d))
+        rng = float(ref.max() - ref.min())
+        rn = float(d.flat[i]) / rng if rng > 0 else float(d.flat[i])
+        at = float(d.flat[i]) / abs(float(ref.flat[i])) if ref.flat[i] != 0 else float("inf")
+        fields[name] = {"max_abs": float(d.flat[i]), "range": rng, "range_normalised": rn,
+                        "rel_at_max_abs_point": at}
+        worst = max(worst, rn)
+        del d
+    ddiag = {}
+    for key in ("uu", "aa", "lnrho"):
+        last64, last32 = diag["fp64"][-1][key], diag["fp32"][-1][key]
+        ddiag[key] = [abs(a - b) / abs(a) if a != 0 else abs(b) for a, b in zip(last64, last32)]
+    return dict(fields=fields, worst_range_normalised=worst, target=1e-5, passed=worst <= 1e-5,
+                diagnostics_rel_diff_last_step=ddiag, s_per_step=times)
 
 
 def copy_probe(double: bool, n: int, reps: int, hip: Hip):
@@ -186,6 +270,10 @@ def main(argv=None):
                     help="force a solve configuration 'variant,tile_x,tile_y,z_depth' instead of autotuning")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="REPS",
                     help="only run REPS launches of the plain copy kernel and print its bandwidth")
+    ap.add_argument("--workload", choices=["headline", "config2", "config3", "config5"], default="headline",
+                    help="headline: the 512^3 FP32 vs FP64 metric (default). config2: 256^3 with per-step "
+                         "FP64 diagnostics and the tuning sweep. config3: 512^3 FP32-vs-FP64 precision "
+                         "study. config5: 1024^3 FP32 with the 26-block pack/unpack halo")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this path has no multi-GPU layer (single periodic box "
@@ -198,6 +286,8 @@ def main(argv=None):
                   flush=True)
         return 0
     kcfg = tuple(int(v) for v in a.kernel_config.split(",")) if a.kernel_config else None
+    if a.workload != "headline":
+        return run_workload(a, kcfg, hip)
     res = {}
     for prec in (["fp32", "fp64"] if a.precision == "both" else [a.precision]):
         res[prec] = run_gpu(prec == "fp64", a.n, a.steps if prec == "fp32" else max(3, a.steps // 2),
@@ -253,6 +343,89 @@ def main(argv=None):
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
     return 0
+
+
+def run_workload(a, kcfg, hip):
+    """The other measured configurations; each prints one JSON line of its own."""
+    base = {"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "data": "synthetic: seeded splitmix64 fields (config-1 distribution)"}
+    if a.workload == "config3":
+        n = a.n
+        steps = 10
+        r = precision_study(n, steps, a.dt, hip)
+        line = dict(base, metric="FP32 vs FP64 range-normalised max error after %d RK3 steps at %d^3" % (steps, n),
+                    value=r["worst_range_normalised"], unit="relative", higher_is_better=False, steps=steps,
+                    warmup=0, dtype="f32 vs f64",
+                    config={"workload": "precision study: config-1 state + u_x 1e-2 sin(2z), %d^3, dt=%g" % (n, a.dt)},
+                    precision=r)
+        line["data"] = "synthetic: seeded splitmix64 config-3 fields (config-1 + u_x 1e-2 sin 2z)"
+        print(json.dumps(line), flush=True)
+        return 0
+    if a.workload == "config5":
+        n = 1024 if a.n == 512 else a.n
+        # The explicit grad-div (bulk + shear) viscosity limits dt ~ dx^2: dt = 1e-3 is inside the
+        # RK3 stability region at 512^3 (nu_eff dt |lambda_max| = 2.0 < 2.5) but not at 1024^3, so
+        # the step is scaled with dx^2 (same margin as 512^3). Throughput does not depend on dt.
+        dt = a.dt * (512.0 / n) ** 2
+        r = run_gpu(False, n, a.steps if a.steps != 20 else 5, 1, dt, False, hip, kcfg, halo="pack",
+                    reduce_each_step=True)
+        line = dict(base, metric="cell-updates/s per RK3 step at %d^3 FP32, 26-block pack/unpack halo" % n,
+                    value=r["value"], unit="cell-updates/s", steps=a.steps if a.steps != 20 else 5, warmup=1,
+                    ms_per_step=r["ms_per_step"], dtype="f32",
+                    config={"workload": "8-field MHD, %d^3 periodic box, dt=%g, halo by packing the 26 ghost "
+                                        "blocks' periodic images and unpacking them (self exchange), FP64 "
+                                        "diagnostics after every step" % (n, dt),
+                            "kernel_config": r["kernel_config"], "median_step_ms": r["median_step_ms"],
+                            "p90_step_ms": r["p90_step_ms"], "solve_fraction_of_step": r["solve_fraction"],
+                            "halo_ms_per_substep": r["halo_ms_mean"]},
+                    roofline={"bound": "hbm", "kernel": "solve (fused MHD stencil + RK3 substep)",
+                              "achieved": r["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": r["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
+                              "solve_ms_per_launch": r["solve_ms_mean"]},
+                    halo={"bytes_per_exchange": r["halo_alg_bytes"], "ms_per_exchange": r["halo_ms_mean"],
+                          "achieved_gbs": r["halo_alg_bytes"] / (r["halo_ms_mean"] * 1e-3) / 1e9},
+                    diagnostics_last_step=r["reductions"][-1] if r["reductions"] else None)
+        print(json.dumps(line), flush=True)
+        return 0
+    if a.workload == "config2":
+        n = 256 if a.n == 512 else a.n
+        out = {}
+        for prec in ("fp32", "fp64"):
+            r = run_gpu(prec == "fp64", n, 10, 2, a.dt, True, hip, kcfg, reduce_each_step=True)
+            out[prec] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "achieved_gbs": r["achieved_gbs"],
+                         "frac": r["achieved_gbs"] / HBM_PEAK_GBS, "kernel_config": r["kernel_config"],
+                         "diagnostics": r["reductions"]}
+            out[prec]["sweep"] = tuning_sweep(prec == "fp64", n)
+        line = dict(base, metric="cell-updates/s per RK3 step at %d^3, FP32 and FP64, per-step FP64 diagnostics" % n,
+                    value=out["fp32"]["value"], unit="cell-updates/s", steps=10, warmup=2,
+                    ms_per_step=out["fp32"]["ms_per_step"], dtype="f32",
+                    config={"workload": "8-field MHD, %d^3 periodic box, autotuned" % n}, results=out)
+        print(json.dumps(line), flush=True)
+        return 0
+    raise ValueError(a.workload)
+
+
+def tuning_sweep(double: bool, n: int):
+    """Mean substep time of every compiled configuration and z-depth (the autotuner's search)."""
+    from astaroth_amd.api import Library
+    from astaroth_amd.abi import KernelConfig
+    from astaroth_amd.initial import device_init_mhd
+
+    lib = Library(double)
+    dev = lib.device_create(0, lib.config(n=n), autotune=False)
+    res = []
+    try:
+        device_init_mhd(dev)
+        for c in lib.kernel_configs():
+            depths = [0] if c.variant == 0 else ([8, 16, 32, 64, 0] if c.variant == 1 else [14, 28, 56, 0])
+            for zd in depths:
+                if zd >= n:
+                    continue
+                cc = KernelConfig(c.variant, c.tile_x, c.tile_y, zd)
+                res.append([list(cc.as_tuple()), dev.time_kernel_config(cc, 3)])
+    finally:
+        dev.destroy()
+    return sorted(res, key=lambda x: x[1])
 
 
 if __name__ == "__main__":

@@ -67,6 +67,14 @@ AcResult acDeviceRandomizeUniform(const Device device, const Stream stream,
                                   const VertexBufferHandle handle, const double lo,
                                   const double hi, const unsigned long long seed);
 
+/* Same stream plus amplitude * sin(phase_per_cell * (k - 3)), evaluated in FP64 before the
+ * single rounding to AcReal. With phase_per_cell = 2 dz this is the 1e-2 sin(2 z) perturbation
+ * of u_x in the 512^3 precision study; dz is passed in FP64 because AC_dsz may be FP32. */
+AcResult acDeviceRandomizeUniformSinZ(const Device device, const Stream stream,
+                                      const VertexBufferHandle handle, const double lo,
+                                      const double hi, const unsigned long long seed,
+                                      const double amplitude, const double phase_per_cell);
+
 /* ---- Plain copy of in[handle] into out[handle] (all m^3 cells), asynchronous on `stream`.
  * One read and one write per cell: the known-size kernel used to calibrate memory counters. */
 AcResult acDeviceCopyVertexBuffer(const Device device, const Stream stream,

@@ -129,6 +129,29 @@ def test_device_uniform_generator_matches_host(lib):
     assert np.array_equal(out.interior(VTX["VTXBUF_UUY"]), ref)
 
 
+def test_device_perturbed_init_matches_host(lib):
+    """device_init_mhd(perturb_sin2z=True) against the host generator of the precision study:
+    u_x = U(-1e-3, 1e-3) + 1e-2 sin(2 z), evaluated in FP64 and rounded once. The device sin
+    (OCML) and the host sin (libm) may differ in the last FP64 bit, so the bar is 1 ulp of
+    AcReal; every other field must match bitwise."""
+    from astaroth_amd.initial import device_init_mhd
+    nx, ny, nz = 16, 10, 24
+    info = lib.config(n=nx, ny=ny, nz=nz)
+    with make_device(lib, info) as dev:
+        device_init_mhd(dev, config_id=3, perturb_sin2z=True)
+        out = HostMesh(lib, info)
+        dev.store_mesh(out)
+    ref = mhd_fields(nx, ny, nz, np.float64, config_id=3, perturb_sin2z=True)
+    for q in range(8):
+        r = ref[q].astype(lib.dtype)
+        if q == VTX["VTXBUF_UUX"]:
+            ulp = np.spacing(np.abs(r).astype(lib.dtype))
+            assert np.all(np.abs(out.fields[q] - r) <= ulp)
+            assert np.abs(r[3:-3, 3:-3, 3:-3]).max() > 5e-3  # the perturbation is there
+        else:
+            assert np.array_equal(out.fields[q], r)
+
+
 def test_kernel_variants_agree(lib):
     """Every compiled solve configuration computes the same substeps (variant 0 is the direct
     stencil evaluation; variant 1 builds cross derivatives by accumulation)."""
