@@ -124,6 +124,22 @@ class Library:
             "acDeviceCopyVertexBuffer": (i, [vp, i, i]),
             "acDeviceRandomizeUniformSinZ": (i, [vp, i, i, ctypes.c_double, ctypes.c_double,
                                                  ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double]),
+            "acEnsembleCreate": (i, [i, i, ctypes.POINTER(MI), ctypes.POINTER(vp)]),
+            "acEnsembleDestroy": (i, [vp]),
+            "acEnsembleBoxCount": (i, [vp]),
+            "acEnsembleSetKernelConfig": (i, [vp, abi.KernelConfig]),
+            "acEnsembleGetKernelConfig": (i, [vp, ctypes.POINTER(abi.KernelConfig)]),
+            "acEnsembleRandomizeUniform": (i, [vp, i, i, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong]),
+            "acEnsembleSetVertexBuffer": (i, [vp, i, i, R]),
+            "acEnsembleLoadMesh": (i, [vp, i, M]),
+            "acEnsembleStoreMesh": (i, [vp, i, ctypes.POINTER(M)]),
+            "acEnsemblePeriodicBoundconds": (i, [vp]),
+            "acEnsembleIntegrateStep": (i, [vp, R]),
+            "acEnsembleIntegrateStepGraph": (i, [vp, R]),
+            "acEnsembleIntegrateStepLooped": (i, [vp, R]),
+            "acEnsembleDiagnostics": (i, [vp, ctypes.POINTER(ctypes.c_double)]),
+            "acEnsembleSynchronize": (i, [vp]),
+            "acEnsembleGetStreamHandle": (vp, [vp]),
             "acDevicePackData": (i, [vp, i, I3, I3, vp]),
             "acDeviceUnpackData": (i, [vp, i, vp, I3, I3]),
             "acDeviceHaloPackedSize": (ctypes.c_size_t, [vp]),
@@ -182,6 +198,9 @@ class Library:
 
     def device_create(self, device_id: int, info, autotune: bool | None = None) -> "Device":
         return Device(self, device_id, info, autotune)
+
+    def ensemble_create(self, infos, device_id: int = 0) -> "Ensemble":
+        return Ensemble(self, infos, device_id)
 
     def kernel_configs(self):
         out = []
@@ -406,6 +425,85 @@ class Device:
     def halo_exchange_self(self, stream=abi.STREAM_DEFAULT):
         return self._check(self.lib.dll.acDeviceHaloExchangeSelf(self.handle, stream),
                            "acDeviceHaloExchangeSelf")
+
+
+class Ensemble:
+    """Handle of acEnsembleCreate: independent periodic boxes advanced together
+    (include/astaroth_ext.h)."""
+
+    DIAGNOSTICS = ("max_u", "rms_u", "max_a", "rms_a", "max_lnrho", "min_lnrho", "rms_lnrho")
+
+    def __init__(self, lib: Library, infos, device_id: int = 0):
+        self.lib = lib
+        self.infos = list(infos)
+        arr = (abi.structs(lib.double)[0] * len(self.infos))(*self.infos)
+        self.handle = ctypes.c_void_p()
+        if lib.dll.acEnsembleCreate(device_id, len(self.infos), arr, ctypes.byref(self.handle)) != abi.AC_SUCCESS:
+            raise AstarothError("acEnsembleCreate failed")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+    def destroy(self):
+        if self.handle:
+            self.lib.dll.acEnsembleDestroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def _check(self, res, what):
+        if res != abi.AC_SUCCESS:
+            raise AstarothError("%s returned AC_FAILURE" % what)
+        return res
+
+    @property
+    def nboxes(self):
+        return len(self.infos)
+
+    def set_kernel_config(self, tile_x, tile_y, z_depth=0):
+        c = abi.KernelConfig(3, tile_x, tile_y, z_depth)
+        return self._check(self.lib.dll.acEnsembleSetKernelConfig(self.handle, c), "acEnsembleSetKernelConfig")
+
+    def kernel_config(self) -> abi.KernelConfig:
+        c = abi.KernelConfig()
+        self._check(self.lib.dll.acEnsembleGetKernelConfig(self.handle, ctypes.byref(c)), "acEnsembleGetKernelConfig")
+        return c
+
+    def randomize_uniform(self, box, handle, lo, hi, seed):
+        return self._check(self.lib.dll.acEnsembleRandomizeUniform(self.handle, box, _vh(handle), lo, hi, seed),
+                           "acEnsembleRandomizeUniform")
+
+    def set_vertex_buffer(self, box, handle, value):
+        return self._check(self.lib.dll.acEnsembleSetVertexBuffer(self.handle, box, _vh(handle), value),
+                           "acEnsembleSetVertexBuffer")
+
+    def load_mesh(self, box, mesh: HostMesh):
+        return self._check(self.lib.dll.acEnsembleLoadMesh(self.handle, box, mesh.struct), "acEnsembleLoadMesh")
+
+    def store_mesh(self, box, mesh: HostMesh):
+        return self._check(self.lib.dll.acEnsembleStoreMesh(self.handle, box, ctypes.byref(mesh.struct)),
+                           "acEnsembleStoreMesh")
+
+    def periodic_boundconds(self):
+        return self._check(self.lib.dll.acEnsemblePeriodicBoundconds(self.handle), "acEnsemblePeriodicBoundconds")
+
+    def integrate_step(self, dt, mode="batched"):
+        fn = {"batched": self.lib.dll.acEnsembleIntegrateStep, "graph": self.lib.dll.acEnsembleIntegrateStepGraph,
+              "looped": self.lib.dll.acEnsembleIntegrateStepLooped}[mode]
+        return self._check(fn(self.handle, self.lib.real(dt)), "acEnsembleIntegrateStep(%s)" % mode)
+
+    def diagnostics(self) -> np.ndarray:
+        out = np.zeros((self.nboxes, 7), dtype=np.float64)
+        self._check(self.lib.dll.acEnsembleDiagnostics(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                    "acEnsembleDiagnostics")
+        return out
+
+    def synchronize(self):
+        return self._check(self.lib.dll.acEnsembleSynchronize(self.handle), "acEnsembleSynchronize")
+
+    def stream_handle(self):
+        return self.lib.dll.acEnsembleGetStreamHandle(self.handle)
 
 
 def _rt(r):

@@ -23,7 +23,8 @@ LIBDIR = os.path.join(PKG, "lib")
 BUILD = os.path.join(REPO, "_build")
 INCLUDE = os.path.join(REPO, "include")
 
-HIP_SOURCES = ["device.hip", "solve.hip", "boundconds.hip", "reductions.hip", "init.hip", "halo.hip"]
+HIP_SOURCES = ["device.hip", "solve.hip", "boundconds.hip", "reductions.hip", "init.hip", "halo.hip",
+               "ensemble.hip"]
 HOST_SOURCES = ["host.cc"]
 ARCH = "gfx950"
 

@@ -74,6 +74,12 @@ struct SubstepArgs {
     RK3Coeffs rk;
 };
 
+// One work item of the batched solve: tile column (btx, bty) of box `box`, output planes
+// [zs, zs + zdepth) clipped to the box.
+struct BatchItem {
+    int box, btx, bty, zs, zdepth;
+};
+
 PhysParams make_phys_params(const AcMeshInfo& info);
 RK3Coeffs make_rk3_coeffs(int step, AcReal dt);
 
@@ -81,6 +87,12 @@ RK3Coeffs make_rk3_coeffs(int step, AcReal dt);
 // one of the compiled configurations.
 hipError_t launch_substep(const AcKernelConfig& config, const SubstepArgs& args, hipStream_t stream);
 bool kernel_config_supported(const AcKernelConfig& config);
+// Batched solve (persistent lean z-march over a work list): `boxes` and `items` are device
+// arrays; tile shapes 32x4, 16x8, 16x16.
+bool batch_tile_supported(int tile_x, int tile_y);
+hipError_t launch_substep_batch(int tile_x, int tile_y, const SubstepArgs* boxes,
+                                const BatchItem* items, int nitems, int nblocks,
+                                hipStream_t stream);
 int kernel_config_count();
 AcKernelConfig kernel_config_at(int index);
 AcKernelConfig default_kernel_config(int nx, int ny, int nz);

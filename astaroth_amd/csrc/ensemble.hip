@@ -1,0 +1,693 @@
+// Batched ensembles: many independent periodic boxes of uneven sizes advanced together.
+//
+// There is no counterpart in the reference, which runs one box per Device; this is the batching
+// pattern of the single-device path. Each box keeps the reference's layout (8 `in` and 8 `out`
+// fields of m_x m_y m_z cells, x fastest) and its own AcMeshInfo (grid spacing, viscosity, ...).
+// One RK3 substep is
+//   * one periodic-fill launch over the ghost shells of all boxes (a flat list of ghost cells),
+//   * one persistent solve launch (solve_zlean_batch) over a (box, tile column, z range) work
+//     list sorted by decreasing cost and dealt round-robin to the resident blocks,
+// and a step ends with per-box FP64 diagnostics (max and sum of squares of |u| and |A|, max, min
+// and sum of squares of lnrho) in two launches. acEnsembleIntegrateStepGraph records the whole
+// step once per (dt, buffer parity) into a HIP graph and replays it.
+#include "ghost.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+using namespace ac;
+
+namespace {
+
+constexpr int NDIAG     = 7;  // per-box diagnostics, see acEnsembleDiagnostics
+constexpr int DIAG_BLKS = 16; // blocks per box of the diagnostics launch
+
+struct FillBox {
+    AcReal* f[NF];
+    GhostShell g;
+    long first; // index of the box's first ghost cell among all boxes
+};
+
+__global__ void __launch_bounds__(256)
+periodic_fill_batch(const FillBox* __restrict__ boxes, const int nboxes, const long total)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total)
+        return;
+    int lo = 0, hi = nboxes - 1; // last box whose first cell is <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (boxes[mid].first <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const FillBox& b = boxes[lo];
+    int i, j, k;
+    b.g.cell(t - b.first, i, j, k);
+    long dst, src;
+    b.g.indices(i, j, k, dst, src);
+#pragma unroll
+    for (int q = 0; q < NF; ++q)
+        b.f[q][dst] = b.f[q][src];
+}
+
+struct DiagBox {
+    const AcReal* f[NF];
+    int nx, ny, nz, mx;
+    long mxy;
+};
+
+__device__ __forceinline__ double
+dmax(double a, double b)
+{
+    return a > b ? a : b;
+}
+__device__ __forceinline__ double
+dmin(double a, double b)
+{
+    return a < b ? a : b;
+}
+
+// Quantity q of the diagnostics: 0 max |u|^2, 1 sum |u|^2, 2 max |A|^2, 3 sum |A|^2,
+// 4 max lnrho, 5 min lnrho, 6 sum lnrho^2 (square roots taken when finalising).
+__device__ __forceinline__ bool
+is_max(int q)
+{
+    return q == 0 || q == 2 || q == 4;
+}
+
+__global__ void __launch_bounds__(256)
+diagnostics_batch(const DiagBox* __restrict__ boxes, double* __restrict__ partials)
+{
+    __shared__ double lds[4][NDIAG];
+    const DiagBox& b = boxes[blockIdx.y];
+    double v[NDIAG]  = {0.0, 0.0, 0.0, 0.0, -INFINITY, INFINITY, 0.0};
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nrows = b.ny * b.nz;
+    for (int r = blockIdx.x * 4 + wave; r < nrows; r += gridDim.x * 4) {
+        const int j = 3 + r % b.ny, k = 3 + r / b.ny;
+        const long row = (long)j * b.mx + (long)k * b.mxy + 3;
+        for (int i = lane; i < b.nx; i += 64) {
+            const long c  = row + i;
+            const double ux = b.f[VTXBUF_UUX][c], uy = b.f[VTXBUF_UUY][c], uz = b.f[VTXBUF_UUZ][c];
+            const double ax = b.f[VTXBUF_AX][c], ay = b.f[VTXBUF_AY][c], az = b.f[VTXBUF_AZ][c];
+            const double lr = b.f[VTXBUF_LNRHO][c];
+            const double u2 = ux * ux + uy * uy + uz * uz, a2 = ax * ax + ay * ay + az * az;
+            v[0] = dmax(v[0], u2);
+            v[1] += u2;
+            v[2] = dmax(v[2], a2);
+            v[3] += a2;
+            v[4] = dmax(v[4], lr);
+            v[5] = dmin(v[5], lr);
+            v[6] += lr * lr;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NDIAG; ++q) {
+        double x = v[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double y = __shfl_xor(x, off, 64);
+            x              = q == 5 ? dmin(x, y) : is_max(q) ? dmax(x, y) : x + y;
+        }
+        if (lane == 0)
+            lds[wave][q] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NDIAG) {
+        const int q = threadIdx.x;
+        double x    = lds[0][q];
+        for (int w = 1; w < 4; ++w)
+            x = q == 5 ? dmin(x, lds[w][q]) : is_max(q) ? dmax(x, lds[w][q]) : x + lds[w][q];
+        partials[((long)blockIdx.y * gridDim.x + blockIdx.x) * NDIAG + q] = x;
+    }
+}
+
+__global__ void
+diagnostics_finalize(const double* __restrict__ partials, const int nboxes, const int nblk,
+                     double* __restrict__ results)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nboxes * NDIAG)
+        return;
+    const int box = t / NDIAG, q = t % NDIAG;
+    double x = partials[((long)box * nblk) * NDIAG + q];
+    for (int i = 1; i < nblk; ++i) {
+        const double y = partials[((long)box * nblk + i) * NDIAG + q];
+        x              = q == 5 ? dmin(x, y) : is_max(q) ? dmax(x, y) : x + y;
+    }
+    results[t] = x;
+}
+
+} // namespace
+
+struct ensemble_s {
+    int id;
+    int nboxes;
+    std::vector<AcMeshInfo> info;
+    std::vector<AcReal*> fields[2][NF]; // [set][field][box]; set 0/1 = in/out at parity 0
+    AcReal* base;
+    size_t base_bytes;
+    hipStream_t stream;
+    int parity; // number of swaps mod 2: set `parity` holds the current state
+    // batched solve
+    int tile_x, tile_y, z_depth, nblocks;
+    BatchItem* d_items;
+    int nitems;
+    SubstepArgs* d_args[3][2]; // per (substep, parity), for dt_args
+    AcReal dt_args;
+    bool args_valid;
+    // batched fill and diagnostics, per parity
+    FillBox* d_fill[2];
+    long fill_total;
+    DiagBox* d_diag[2];
+    double* d_partials;
+    double* d_results;
+    double* h_results; // pinned
+    std::map<unsigned long long, hipGraphExec_t> graphs[2];
+};
+
+namespace {
+
+int3
+dims_of(const AcMeshInfo& i)
+{
+    return make_int3(i.int_params[AC_nx], i.int_params[AC_ny], i.int_params[AC_nz]);
+}
+
+size_t
+cells_of(const AcMeshInfo& i)
+{
+    const int3 n = dims_of(i);
+    return (size_t)(n.x + 6) * (n.y + 6) * (n.z + 6);
+}
+
+bool
+ens_ok(const Ensemble e, int box)
+{
+    return e && box >= 0 && box < e->nboxes;
+}
+
+void
+drop_graphs(Ensemble e)
+{
+    for (int p = 0; p < 2; ++p) {
+        for (auto& kv : e->graphs[p])
+            (void)hipGraphExecDestroy(kv.second);
+        e->graphs[p].clear();
+    }
+}
+
+AcResult
+build_items(Ensemble e)
+{
+    struct Item {
+        BatchItem w;
+        long cost;
+    };
+    std::vector<Item> items;
+    const int TX = e->tile_x, TY = e->tile_y;
+    for (int b = 0; b < e->nboxes; ++b) {
+        const int3 n = dims_of(e->info[b]);
+        const int zd = (e->z_depth <= 0 || e->z_depth >= n.z) ? n.z : e->z_depth;
+        const int ntx = (n.x + TX - 1) / TX, nty = (n.y + TY - 1) / TY, ntz = (n.z + zd - 1) / zd;
+        for (int tz = 0; tz < ntz; ++tz) {
+            const int zs    = NGHOST + tz * zd;
+            const int depth = std::min(zd, n.z + NGHOST - zs);
+            // planes marched: 6 warm-up planes plus the outputs rounded up to 7-plane rounds
+            const long cost = 6 + ((depth + 6) / 7) * 7;
+            for (int ty = 0; ty < nty; ++ty)
+                for (int tx = 0; tx < ntx; ++tx)
+                    items.push_back({{b, tx, ty, zs, zd}, cost});
+        }
+    }
+    std::stable_sort(items.begin(), items.end(),
+                     [](const Item& a, const Item& b) { return a.cost > b.cost; });
+    std::vector<BatchItem> w(items.size());
+    for (size_t i = 0; i < items.size(); ++i)
+        w[i] = items[i].w;
+    if (e->d_items)
+        AC_HIP_CHECK(hipFree(e->d_items));
+    e->d_items = nullptr;
+    e->nitems  = (int)w.size();
+    if (!w.empty()) {
+        AC_HIP_CHECK(hipMalloc((void**)&e->d_items, w.size() * sizeof(BatchItem)));
+        AC_HIP_CHECK(hipMemcpy(e->d_items, w.data(), w.size() * sizeof(BatchItem),
+                               hipMemcpyHostToDevice));
+    }
+    // resident block slots of the persistent grid: one wave per SIMD
+    int cus = 0;
+    AC_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->id));
+    const int waves_per_block = (TX * TY + 63) / 64;
+    e->nblocks = std::max(1, cus * 4 / waves_per_block);
+    return AC_SUCCESS;
+}
+
+AcResult
+build_args(Ensemble e, AcReal dt)
+{
+    drop_graphs(e); // recorded graphs read these argument arrays
+    AC_HIP_CHECK(hipStreamSynchronize(e->stream));
+    // (a plain array: std::vector cannot relocate a struct of __restrict__ pointer arrays)
+    std::unique_ptr<SubstepArgs[]> a(new SubstepArgs[e->nboxes]);
+    for (int s = 0; s < 3; ++s)
+        for (int par = 0; par < 2; ++par) {
+            // substep s at buffer parity `par` reads set `par` and writes the other set
+            for (int b = 0; b < e->nboxes; ++b) {
+                const int3 n    = dims_of(e->info[b]);
+                SubstepArgs& x  = a[b];
+                for (int q = 0; q < NF; ++q) {
+                    x.in[q]  = e->fields[par][q][b];
+                    x.out[q] = e->fields[1 - par][q][b];
+                }
+                x.mx    = n.x + 6;
+                x.my    = n.y + 6;
+                x.mz    = n.z + 6;
+                x.mxy   = x.mx * x.my;
+                x.start = make_int3(NGHOST, NGHOST, NGHOST);
+                x.end   = make_int3(n.x + NGHOST, n.y + NGHOST, n.z + NGHOST);
+                x.phys  = make_phys_params(e->info[b]);
+                x.rk    = make_rk3_coeffs(s, dt);
+            }
+            AC_HIP_CHECK(hipMemcpy(e->d_args[s][par], a.get(), e->nboxes * sizeof(SubstepArgs),
+                                   hipMemcpyHostToDevice));
+        }
+    e->dt_args    = dt;
+    e->args_valid = true;
+    return AC_SUCCESS;
+}
+
+AcResult
+ensure_args(Ensemble e, AcReal dt)
+{
+    if (e->args_valid && e->dt_args == dt)
+        return AC_SUCCESS;
+    return build_args(e, dt);
+}
+
+hipError_t
+enqueue_fill(Ensemble e, int par)
+{
+    if (e->fill_total == 0)
+        return hipSuccess;
+    const long blocks = (e->fill_total + 255) / 256;
+    periodic_fill_batch<<<dim3((unsigned)blocks), dim3(256), 0, e->stream>>>(e->d_fill[par],
+                                                                              e->nboxes,
+                                                                              e->fill_total);
+    return hipGetLastError();
+}
+
+hipError_t
+enqueue_diagnostics(Ensemble e, int par)
+{
+    diagnostics_batch<<<dim3(DIAG_BLKS, e->nboxes), dim3(256), 0, e->stream>>>(e->d_diag[par],
+                                                                               e->d_partials);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess)
+        return err;
+    const int n = e->nboxes * NDIAG;
+    diagnostics_finalize<<<dim3((n + 255) / 256), dim3(256), 0, e->stream>>>(
+        e->d_partials, e->nboxes, DIAG_BLKS, e->d_results);
+    return hipGetLastError();
+}
+
+// The step at the current parity, enqueued on the ensemble stream (also what a graph records).
+AcResult
+enqueue_step(Ensemble e, int par0)
+{
+    int par = par0;
+    for (int s = 0; s < 3; ++s) {
+        AC_HIP_CHECK(enqueue_fill(e, par));
+        AC_HIP_CHECK(launch_substep_batch(e->tile_x, e->tile_y, e->d_args[s][par], e->d_items,
+                                          e->nitems, e->nblocks, e->stream));
+        par = 1 - par;
+    }
+    AC_HIP_CHECK(enqueue_diagnostics(e, par));
+    return AC_SUCCESS;
+}
+
+bool
+valid_dt(AcReal dt)
+{
+    return std::isfinite((double)dt) && dt > 0;
+}
+
+} // namespace
+
+extern "C" {
+
+AcResult
+acEnsembleCreate(const int device_id, const int nboxes, const AcMeshInfo* configs,
+                 Ensemble* ensemble)
+{
+    if (!ensemble || !configs || nboxes < 1)
+        return AC_FAILURE;
+    *ensemble = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device_id < 0 || device_id >= count) {
+        fprintf(stderr, "acEnsembleCreate: no HIP device %d\n", device_id);
+        return AC_FAILURE;
+    }
+    for (int b = 0; b < nboxes; ++b) {
+        const int3 n = dims_of(configs[b]);
+        if (n.x < 1 || n.y < 1 || n.z < 1) {
+            fprintf(stderr, "acEnsembleCreate: box %d has an empty grid\n", b);
+            return AC_FAILURE;
+        }
+    }
+    AC_HIP_CHECK(hipSetDevice(device_id));
+    Ensemble e   = new ensemble_s();
+    e->id        = device_id;
+    e->nboxes    = nboxes;
+    e->info.assign(configs, configs + nboxes);
+    for (auto& i : e->info)
+        acHostUpdateBuiltinParams(&i);
+    e->parity   = 0;
+    e->tile_x   = 32;
+    e->tile_y   = 4;
+    e->z_depth  = 0;
+    e->d_items  = nullptr;
+    e->args_valid = false;
+    e->dt_args  = 0;
+    // one allocation: box after box, 16 fields each (8 per set), 256-byte aligned
+    size_t bytes = 0;
+    std::vector<size_t> offs(nboxes);
+    for (int b = 0; b < nboxes; ++b) {
+        offs[b] = bytes;
+        const size_t fb = (cells_of(e->info[b]) * sizeof(AcReal) + 255) / 256 * 256;
+        bytes += 2 * NF * fb;
+    }
+    e->base_bytes = bytes;
+    AC_HIP_CHECK(hipMalloc((void**)&e->base, bytes));
+    AC_HIP_CHECK(hipMemset(e->base, 0, bytes));
+    for (int set = 0; set < 2; ++set)
+        for (int q = 0; q < NF; ++q)
+            e->fields[set][q].resize(nboxes);
+    for (int b = 0; b < nboxes; ++b) {
+        const size_t fb = (cells_of(e->info[b]) * sizeof(AcReal) + 255) / 256 * 256;
+        for (int set = 0; set < 2; ++set)
+            for (int q = 0; q < NF; ++q)
+                e->fields[set][q][b] = (AcReal*)((char*)e->base + offs[b] + (set * NF + q) * fb);
+    }
+    AC_HIP_CHECK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    for (int s = 0; s < 3; ++s)
+        for (int p = 0; p < 2; ++p)
+            AC_HIP_CHECK(hipMalloc((void**)&e->d_args[s][p], nboxes * sizeof(SubstepArgs)));
+    // fill and diagnostics descriptors per parity
+    std::vector<FillBox> fb(nboxes);
+    std::vector<DiagBox> db(nboxes);
+    for (int p = 0; p < 2; ++p) {
+        AC_HIP_CHECK(hipMalloc((void**)&e->d_fill[p], nboxes * sizeof(FillBox)));
+        AC_HIP_CHECK(hipMalloc((void**)&e->d_diag[p], nboxes * sizeof(DiagBox)));
+        long first = 0;
+        for (int b = 0; b < nboxes; ++b) {
+            const int3 n = dims_of(e->info[b]);
+            for (int q = 0; q < NF; ++q) {
+                fb[b].f[q] = e->fields[p][q][b];
+                db[b].f[q] = e->fields[p][q][b];
+            }
+            fb[b].g     = GhostShell::of(n.x, n.y, n.z);
+            fb[b].first = first;
+            first += fb[b].g.cells();
+            db[b].nx  = n.x;
+            db[b].ny  = n.y;
+            db[b].nz  = n.z;
+            db[b].mx  = n.x + 6;
+            db[b].mxy = (long)(n.x + 6) * (n.y + 6);
+        }
+        e->fill_total = first;
+        AC_HIP_CHECK(hipMemcpy(e->d_fill[p], fb.data(), nboxes * sizeof(FillBox),
+                               hipMemcpyHostToDevice));
+        AC_HIP_CHECK(hipMemcpy(e->d_diag[p], db.data(), nboxes * sizeof(DiagBox),
+                               hipMemcpyHostToDevice));
+    }
+    AC_HIP_CHECK(hipMalloc((void**)&e->d_partials, (size_t)nboxes * DIAG_BLKS * NDIAG * sizeof(double)));
+    AC_HIP_CHECK(hipMalloc((void**)&e->d_results, (size_t)nboxes * NDIAG * sizeof(double)));
+    AC_HIP_CHECK(hipMemset(e->d_results, 0, (size_t)nboxes * NDIAG * sizeof(double)));
+    AC_HIP_CHECK(hipHostMalloc((void**)&e->h_results, (size_t)nboxes * NDIAG * sizeof(double)));
+    if (build_items(e) != AC_SUCCESS)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipDeviceSynchronize());
+    *ensemble = e;
+    return AC_SUCCESS;
+}
+
+AcResult
+acEnsembleDestroy(Ensemble e)
+{
+    if (!e)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(e->id));
+    AC_HIP_CHECK(hipStreamSynchronize(e->stream));
+    drop_graphs(e);
+    AC_HIP_CHECK(hipFree(e->base));
+    for (int s = 0; s < 3; ++s)
+        for (int p = 0; p < 2; ++p)
+            AC_HIP_CHECK(hipFree(e->d_args[s][p]));
+    for (int p = 0; p < 2; ++p) {
+        AC_HIP_CHECK(hipFree(e->d_fill[p]));
+        AC_HIP_CHECK(hipFree(e->d_diag[p]));
+    }
+    if (e->d_items)
+        AC_HIP_CHECK(hipFree(e->d_items));
+    AC_HIP_CHECK(hipFree(e->d_partials));
+    AC_HIP_CHECK(hipFree(e->d_results));
+    AC_HIP_CHECK(hipHostFree(e->h_results));
+    AC_HIP_CHECK(hipStreamDestroy(e->stream));
+    delete e;
+    return AC_SUCCESS;
+}
+
+int
+acEnsembleBoxCount(const Ensemble e)
+{
+    return e ? e->nboxes : 0;
+}
+
+AcResult
+acEnsembleSetKernelConfig(Ensemble e, const AcKernelConfig config)
+{
+    if (!e || config.variant != 3 || !batch_tile_supported(config.tile_x, config.tile_y) ||
+        config.z_depth < 0)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(e->id));
+    AC_HIP_CHECK(hipStreamSynchronize(e->stream));
+    e->tile_x  = config.tile_x;
+    e->tile_y  = config.tile_y;
+    e->z_depth = config.z_depth;
+    drop_graphs(e);
+    return build_items(e);
+}
+
+AcResult
+acEnsembleGetKernelConfig(const Ensemble e, AcKernelConfig* config)
+{
+    if (!e || !config)
+        return AC_FAILURE;
+    config->variant = 3;
+    config->tile_x  = e->tile_x;
+    config->tile_y  = e->tile_y;
+    config->z_depth = e->z_depth;
+    return AC_SUCCESS;
+}
+
+AcResult
+acEnsembleRandomizeUniform(Ensemble e, const int box, const VertexBufferHandle handle,
+                           const double lo, const double hi, const unsigned long long seed)
+{
+    if (!ens_ok(e, box) || handle < 0 || handle >= NF)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(e->id));
+    AC_HIP_CHECK(launch_fill_uniform(e->fields[e->parity][handle][box], e->info[box], lo, hi,
+                                     seed, e->stream));
+    return AC_SUCCESS;
+}
+
+AcResult
+acEnsembleSetVertexBuffer(Ensemble e, const int box, const VertexBufferHandle handle,
+                          const AcReal value)
+{
+    if (!ens_ok(e, box) || handle < 0 || handle >= NF || !std::isfinite((double)value))
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(e->id));
+    const long n = (long)cells_of(e->info[box]);
+    for (int set = 0; set < 2; ++set)
+        AC_HIP_CHECK(launch_fill_constant(e->fields[set][handle][box], n, value, e->stream));
+    return AC_SUCCESS;
+}
+
+AcResult
+acEnsembleLoadMesh(Ensemble e, const int box, const AcMesh host)
+{
+    if (!ens_ok(e, box))
+        return AC_FAILURE;
+    const int3 n = dims_of(host.info);
+    if (n.x != dims_of(e->info[box]).x || n.y != dims_of(e->info[box]).y ||
+        n.z != dims_of(e->info[box]).z)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(e->id));
+    const size_t bytes = cells_of(e->info[box]) * sizeof(AcReal);
+    for (int q = 0; q < NF; ++q)
+        AC_HIP_CHECK(hipMemcpyAsync(e->fields[e->parity][q][box], host.vertex_buffer[q], bytes,
+                                    hipMemcpyHostToDevice, e->stream));
+    AC_HIP_CHECK(hipStreamSynchronize(e->stream));
+    return AC_SUCCESS;
+}
+
+AcResult
+acEnsembleStoreMesh(Ensemble e, const int box, AcMesh* host)
+{
+    if (!ens_ok(e, box) || !host)
+        return AC_FAILURE;
+    const int3 n = dims_of(host->info);
+    if (n.x != dims_of(e->info[box]).x || n.y != dims_of(e->info[box]).y ||
+        n.z != dims_of(e->info[box]).z)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(e->id));
+    const size_t bytes = cells_of(e->info[box]) * sizeof(AcReal);
+    for (int q = 0; q < NF; ++q)
+        AC_HIP_CHECK(hipMemcpyAsync(host->vertex_buffer[q], e->fields[e->parity][q][box], bytes,
+                                    hipMemcpyDeviceToHost, e->stream));
+    AC_HIP_CHECK(hipStreamSynchronize(e->stream));
+    return AC_SUCCESS;
+}
+
+AcResult
+acEnsemblePeriodicBoundconds(Ensemble e)
+{
+    if (!e)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(e->id));
+    AC_HIP_CHECK(enqueue_fill(e, e->parity));
+    return AC_SUCCESS;
+}
+
+AcResult
+acEnsembleIntegrateStep(Ensemble e, const AcReal dt)
+{
+    if (!e || !valid_dt(dt))
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(e->id));
+    if (ensure_args(e, dt) != AC_SUCCESS || enqueue_step(e, e->parity) != AC_SUCCESS)
+        return AC_FAILURE;
+    e->parity = 1 - e->parity;
+    return AC_SUCCESS;
+}
+
+AcResult
+acEnsembleIntegrateStepGraph(Ensemble e, const AcReal dt)
+{
+    if (!e || !valid_dt(dt))
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(e->id));
+    if (ensure_args(e, dt) != AC_SUCCESS) // a new dt rebuilds the arguments and drops the graphs
+        return AC_FAILURE;
+    unsigned long long key = 0;
+    memcpy(&key, &dt, sizeof(dt));
+    auto& cache = e->graphs[e->parity];
+    auto it     = cache.find(key);
+    if (it == cache.end()) {
+        AC_HIP_CHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        const AcResult r = enqueue_step(e, e->parity);
+        hipGraph_t graph;
+        AC_HIP_CHECK(hipStreamEndCapture(e->stream, &graph));
+        if (r != AC_SUCCESS)
+            return AC_FAILURE;
+        hipGraphExec_t exec;
+        AC_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        AC_HIP_CHECK(hipGraphDestroy(graph));
+        it = cache.emplace(key, exec).first;
+    }
+    AC_HIP_CHECK(hipGraphLaunch(it->second, e->stream));
+    e->parity = 1 - e->parity;
+    return AC_SUCCESS;
+}
+
+AcResult
+acEnsembleIntegrateStepLooped(Ensemble e, const AcReal dt)
+{
+    if (!e || !valid_dt(dt))
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(e->id));
+    int par = e->parity;
+    for (int s = 0; s < 3; ++s) {
+        for (int b = 0; b < e->nboxes; ++b) {
+            const int3 n = dims_of(e->info[b]);
+            AcReal* cin[NF];
+            AcReal* cout[NF];
+            for (int q = 0; q < NF; ++q) {
+                cin[q]  = e->fields[par][q][b];
+                cout[q] = e->fields[1 - par][q][b];
+            }
+            const int3 zero = make_int3(0, 0, 0), m = make_int3(n.x + 6, n.y + 6, n.z + 6);
+            AC_HIP_CHECK(launch_periodic_fill(cin, NF, n.x, n.y, n.z, zero, m, e->stream));
+            SubstepArgs a;
+            for (int q = 0; q < NF; ++q) {
+                a.in[q]  = cin[q];
+                a.out[q] = cout[q];
+            }
+            a.mx    = n.x + 6;
+            a.my    = n.y + 6;
+            a.mz    = n.z + 6;
+            a.mxy   = a.mx * a.my;
+            a.start = make_int3(NGHOST, NGHOST, NGHOST);
+            a.end   = make_int3(n.x + NGHOST, n.y + NGHOST, n.z + NGHOST);
+            a.phys  = make_phys_params(e->info[b]);
+            a.rk    = make_rk3_coeffs(s, dt);
+            AC_HIP_CHECK(launch_substep(default_kernel_config(n.x, n.y, n.z), a, e->stream));
+        }
+        par = 1 - par;
+    }
+    AC_HIP_CHECK(enqueue_diagnostics(e, par));
+    e->parity = par;
+    return AC_SUCCESS;
+}
+
+AcResult
+acEnsembleDiagnostics(Ensemble e, double* results)
+{
+    if (!e || !results)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(e->id));
+    const size_t n = (size_t)e->nboxes * NDIAG;
+    AC_HIP_CHECK(hipMemcpyAsync(e->h_results, e->d_results, n * sizeof(double),
+                                hipMemcpyDeviceToHost, e->stream));
+    AC_HIP_CHECK(hipStreamSynchronize(e->stream));
+    for (int b = 0; b < e->nboxes; ++b) {
+        const int3 nn   = dims_of(e->info[b]);
+        const double N  = (double)nn.x * nn.y * nn.z;
+        const double* r = e->h_results + (size_t)b * NDIAG;
+        double* o       = results + (size_t)b * NDIAG;
+        o[0] = std::sqrt(r[0]);     // max |u|
+        o[1] = std::sqrt(r[1] / N); // rms |u|
+        o[2] = std::sqrt(r[2]);     // max |A|
+        o[3] = std::sqrt(r[3] / N); // rms |A|
+        o[4] = r[4];                // max lnrho
+        o[5] = r[5];                // min lnrho
+        o[6] = std::sqrt(r[6] / N); // rms lnrho
+    }
+    return AC_SUCCESS;
+}
+
+AcResult
+acEnsembleSynchronize(const Ensemble e)
+{
+    if (!e)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(e->id));
+    AC_HIP_CHECK(hipStreamSynchronize(e->stream));
+    return AC_SUCCESS;
+}
+
+void*
+acEnsembleGetStreamHandle(const Ensemble e)
+{
+    return e ? (void*)e->stream : nullptr;
+}
+
+} // extern "C"

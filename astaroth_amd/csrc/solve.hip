@@ -833,21 +833,17 @@ template <int TX, int TY> struct ZMarchLean {
     }
 };
 
-template <int TX, int TY, int W>
-__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W)))
-solve_zlean(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
+// One z-march column: tile (btx, bty) of box `a`, output planes [zs, min(zs + zdepth, end.z)).
+template <int TX, int TY>
+__device__ __forceinline__ void
+zlean_column(const SubstepArgs& a, typename ZMarchLean<TX, TY>::Shared& sh, const int btx,
+             const int bty, const int zs, const int zdepth)
 {
     typedef ZMarchLean<TX, TY> K;
-    __shared__ typename K::Shared sh;
     K k(a, sh);
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
-    const int b   = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    const int btx = b % ntx;
-    const int bty = (b / ntx) % nty;
-    const int btz = b / (ntx * nty);
     const int x0 = a.start.x + btx * TX, y0 = a.start.y + bty * TY;
-    const int zs = a.start.z + btz * zdepth;
     k.tid        = tid;
     k.ze         = min(zs + zdepth, a.end.z);
     k.zmax       = a.mz - 1;
@@ -894,6 +890,33 @@ solve_zlean(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
         k.template step<3, true>(st, p + 4);
         k.template step<4, true>(st, p + 5);
         k.template step<5, true>(st, p + 6);
+    }
+}
+
+template <int TX, int TY, int W>
+__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W)))
+solve_zlean(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
+{
+    __shared__ typename ZMarchLean<TX, TY>::Shared sh;
+    const int b   = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int btx = b % ntx;
+    const int bty = (b / ntx) % nty;
+    const int btz = b / (ntx * nty);
+    zlean_column<TX, TY>(a, sh, btx, bty, a.start.z + btz * zdepth, zdepth);
+}
+
+// Persistent form for ensembles of independent boxes: a grid sized to the resident block slots
+// walks a (box, tile column, z range) work list; each box brings its own arguments (pointers,
+// sizes, physical parameters), read from global memory.
+template <int TX, int TY>
+__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(1)))
+solve_zlean_batch(const SubstepArgs* __restrict__ boxes, const BatchItem* __restrict__ items,
+                  const int nitems)
+{
+    __shared__ typename ZMarchLean<TX, TY>::Shared sh;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const BatchItem w = items[it];
+        zlean_column<TX, TY>(boxes[w.box], sh, w.btx, w.bty, w.zs, w.zdepth);
     }
 }
 
@@ -1065,6 +1088,31 @@ launch_substep(const AcKernelConfig& c, const SubstepArgs& a, hipStream_t stream
 #undef AC_ZMARCH
 #undef AC_ZMARCH2
     return hipErrorInvalidConfiguration;
+}
+
+bool
+batch_tile_supported(int tx, int ty)
+{
+    return (tx == 32 && ty == 4) || (tx == 16 && ty == 8) || (tx == 16 && ty == 16);
+}
+
+hipError_t
+launch_substep_batch(int tx, int ty, const SubstepArgs* boxes, const BatchItem* items, int nitems,
+                     int nblocks, hipStream_t stream)
+{
+    if (nitems <= 0)
+        return hipSuccess;
+    if (nblocks <= 0 || nblocks > nitems)
+        nblocks = nitems;
+    if (tx == 32 && ty == 4)
+        solve_zlean_batch<32, 4><<<dim3(nblocks), dim3(128), 0, stream>>>(boxes, items, nitems);
+    else if (tx == 16 && ty == 8)
+        solve_zlean_batch<16, 8><<<dim3(nblocks), dim3(128), 0, stream>>>(boxes, items, nitems);
+    else if (tx == 16 && ty == 16)
+        solve_zlean_batch<16, 16><<<dim3(nblocks), dim3(256), 0, stream>>>(boxes, items, nitems);
+    else
+        return hipErrorInvalidConfiguration;
+    return hipGetLastError();
 }
 
 } // namespace ac

@@ -83,3 +83,15 @@ def device_init_mhd(dev, config_id=1, box_id=0, perturb_sin2z=False):
             dev.randomize_uniform(q, lo, hi, seed_for(config_id, box_id, q))
     dev.periodic_boundconds()
     dev.synchronize()
+
+
+def ensemble_init_mhd(ens, box, config_id=4, box_id=None):
+    """device_init_mhd for one box of an Ensemble (seeds from (config_id, box_id, field));
+    call ens.periodic_boundconds() once all boxes are set."""
+    box_id = box if box_id is None else box_id
+    for name, (lo, hi) in FIELD_RANGES.items():
+        q = VTX[name]
+        if hi == lo:
+            ens.set_vertex_buffer(box, q, lo)
+        else:
+            ens.randomize_uniform(box, q, lo, hi, seed_for(config_id, box_id, q))

@@ -106,6 +106,42 @@ AcResult acDeviceUnpackHalo(const Device device, const Stream stream, const AcRe
  * cell exactly as acDevicePeriodicBoundconds over the whole m^3 box does. */
 AcResult acDeviceHaloExchangeSelf(const Device device, const Stream stream);
 
+/* ---- Batched ensembles of independent periodic boxes ----
+ * No counterpart in the reference (one box per Device there); for callers that advance many
+ * small boxes of uneven sizes, e.g. parameter scans. Each box has its own AcMeshInfo (grid size,
+ * spacing, viscosity, ...) and the reference layout. One RK3 step is 3 x {one periodic-fill launch
+ * over all boxes, one persistent solve launch over a (box, tile column, z range) work list} plus
+ * per-box FP64 diagnostics, on the ensemble's own stream. All boxes share dt.
+ * Box fields are set with acEnsembleRandomizeUniform / acEnsembleSetVertexBuffer (device) or
+ * acEnsembleLoadMesh (host, synchronous); acEnsemblePeriodicBoundconds fills the ghosts. */
+typedef struct ensemble_s* Ensemble;
+AcResult acEnsembleCreate(const int device_id, const int nboxes, const AcMeshInfo* configs,
+                          Ensemble* ensemble);
+AcResult acEnsembleDestroy(Ensemble ensemble);
+int acEnsembleBoxCount(const Ensemble ensemble);
+/* Batched solve configuration: variant 3, tile 32x4, 16x8 or 16x16, z_depth (0: whole box). */
+AcResult acEnsembleSetKernelConfig(Ensemble ensemble, const AcKernelConfig config);
+AcResult acEnsembleGetKernelConfig(const Ensemble ensemble, AcKernelConfig* config);
+AcResult acEnsembleRandomizeUniform(Ensemble ensemble, const int box,
+                                    const VertexBufferHandle handle, const double lo,
+                                    const double hi, const unsigned long long seed);
+AcResult acEnsembleSetVertexBuffer(Ensemble ensemble, const int box,
+                                   const VertexBufferHandle handle, const AcReal value);
+AcResult acEnsembleLoadMesh(Ensemble ensemble, const int box, const AcMesh host);
+AcResult acEnsembleStoreMesh(Ensemble ensemble, const int box, AcMesh* host);
+AcResult acEnsemblePeriodicBoundconds(Ensemble ensemble);
+/* One RK3 step of every box: eager launches, or one HIP graph per (dt, buffer parity) replayed. */
+AcResult acEnsembleIntegrateStep(Ensemble ensemble, const AcReal dt);
+AcResult acEnsembleIntegrateStepGraph(Ensemble ensemble, const AcReal dt);
+/* The same step as a loop of single-box launches (fill + solve per box and substep, each box with
+ * its single-box kernel configuration): the baseline the batched step is measured against. */
+AcResult acEnsembleIntegrateStepLooped(Ensemble ensemble, const AcReal dt);
+/* Diagnostics of the last step, 7 per box: max |u|, rms |u|, max |A|, rms |A|, max lnrho,
+ * min lnrho, rms lnrho (FP64, over the interior; synchronous). */
+AcResult acEnsembleDiagnostics(Ensemble ensemble, double* results);
+AcResult acEnsembleSynchronize(const Ensemble ensemble);
+void* acEnsembleGetStreamHandle(const Ensemble ensemble);
+
 /* ---- Stream access for timing from the caller (returns the hipStream_t as void*) ---- */
 void* acDeviceGetStreamHandle(const Device device, const Stream stream);
 
