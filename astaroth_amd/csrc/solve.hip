@@ -833,6 +833,358 @@ template <int TX, int TY> struct ZMarchLean {
     }
 };
 
+#if AC_DOUBLE_PRECISION == 0
+// ------------------------------------------------------------------------------------------
+// variants 5 and 6 (FP32): the lean z-march with a cell-major LDS ring
+// ------------------------------------------------------------------------------------------
+// The ring stores each cell's eight values as two float4 records (in two rings), so one
+// ds_read_b128 brings four fields of a stencil point and all fields of a plane share two LDS
+// base addresses per ring slot. This halves the LDS instructions of variant 3 and removes its
+// per-field LDS address arithmetic. Group A = (u_x, A_x, u_y, A_y): the x-y cross derivatives
+// need all of it, the x-z sums its low half and the y-z sums its high half; group
+// B = (u_z, A_z, lnrho, s): the cross sums need its low half.
+struct AosCell {
+    float4 A; // ux, ax, uy, ay
+    float4 B; // uz, az, lnrho, s
+};
+// LDS position of each VertexBufferHandle (lnrho, ux, uy, uz, ax, ay, az, s)
+constexpr int AOS_POS[NF] = {6, 0, 2, 4, 1, 3, 5, 7};
+
+__device__ __forceinline__ float
+cell_get(const AosCell& c, int k)
+{
+    switch (k) {
+    case 0: return c.A.x;
+    case 1: return c.A.y;
+    case 2: return c.A.z;
+    case 3: return c.A.w;
+    case 4: return c.B.x;
+    case 5: return c.B.y;
+    case 6: return c.B.z;
+    default: return c.B.w;
+    }
+}
+
+template <int TX, int TY> struct ZMarchAos {
+    static constexpr int NT  = TX * TY;
+    static constexpr int LX  = TX + 6, LY = TY + 6, LN = LX * LY;
+    static constexpr int R   = (LN + NT - 1) / NT;
+    static constexpr int LNP = R * NT;
+
+    // Two rings of float4 (group A, group B): consecutive lanes read consecutive 16-byte
+    // records, which keeps ds_read_b128 free of bank conflicts.
+    struct Shared {
+        float4 ra[4][LNP];
+        float4 rb[4][LNP];
+    };
+
+    struct CellPtr {
+        const float4* __restrict__ a;
+        const float4* __restrict__ b;
+        __device__ __forceinline__ AosCell operator[](int d) const
+        {
+            AosCell c;
+            c.A = a[d];
+            c.B = b[d];
+            return c;
+        }
+    };
+
+    __device__ __forceinline__ CellPtr cells(int slot) const
+    {
+        CellPtr c;
+        c.a = &sh.ra[slot][cofs];
+        c.b = &sh.rb[slot][cofs];
+        return c;
+    }
+
+    struct State {
+        float qv[NF][3]; // planes z-3..z-1, in LDS order
+        float x[6][7];   // cross-derivative sums, slot (z - pbeg) mod 7
+        float pv[NF][R]; // staging registers of the next plane (VertexBufferHandle order)
+    };
+
+    const SubstepArgs& a;
+    Shared& sh;
+    int tid, cofs, ze, zmax;
+    unsigned own;
+    bool active;
+    unsigned goff[R];
+
+    __device__ __forceinline__ ZMarchAos(const SubstepArgs& args, Shared& shared) : a(args), sh(shared) {}
+
+    static __device__ __forceinline__ unsigned pin(unsigned off)
+    {
+        asm volatile("" : "+v"(off));
+        return off;
+    }
+
+    static __device__ __forceinline__ float ld(const float* base, unsigned byte_off)
+    {
+        return *(const float*)((const char*)base + byte_off);
+    }
+
+    __device__ __forceinline__ void issue_loads(State& st, int p) const
+    {
+        const long pl = (long)min(p, zmax) * a.mxy;
+        unsigned go[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            go[r] = pin(goff[r]);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const float* src = a.in[f] + pl;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                st.pv[f][r] = ld(src, go[r]);
+        }
+    }
+
+    __device__ __forceinline__ void push_queue(State& st, const AosCell& c) const
+    {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            st.qv[k][0] = st.qv[k][1];
+            st.qv[k][1] = st.qv[k][2];
+            st.qv[k][2] = cell_get(c, k);
+        }
+    }
+
+    template <int PH, bool OUT> __device__ __forceinline__ void step(State& st, int p) const
+    {
+        const PhysParams& P = a.phys;
+        const int slot      = p & 3;
+
+        __syncthreads(); // every thread is done with ring slot `slot` (plane p - 4)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            sh.ra[slot][r * NT + tid] = make_float4(st.pv[VTXBUF_UUX][r], st.pv[VTXBUF_AX][r],
+                                                    st.pv[VTXBUF_UUY][r], st.pv[VTXBUF_AY][r]);
+            sh.rb[slot][r * NT + tid] = make_float4(st.pv[VTXBUF_UUZ][r], st.pv[VTXBUF_AZ][r],
+                                                    st.pv[VTXBUF_LNRHO][r],
+                                                    st.pv[VTXBUF_ENTROPY][r]);
+        }
+        __syncthreads();
+        issue_loads(st, p + 1);
+
+        const int z            = p - 3;
+        const unsigned long oz = (unsigned long)min(z, ze - 1) * a.mxy;
+        const unsigned ow      = pin(own);
+        float prev[NF];
+        if (OUT && a.rk.step > 0) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                prev[f] = ld(a.out[f] + oz, ow);
+        }
+
+        // ---- ingest plane p: x-z and y-z differences into the pending sums ----
+        const CellPtr c3 = cells(slot);
+#pragma unroll
+        for (int i = 1; i <= 3; ++i) {
+            const int lo = (PH - i + 7) % 7, hi = (PH + i) % 7; // outputs p - i and p + i
+            const float cx = P.cxz[i], cy = P.cyz[i];
+            const float2 xpA = *(const float2*)&c3.a[i];            // ux, ax at x + i
+            const float2 xmA = *(const float2*)&c3.a[-i];
+            const float2 xpB = *(const float2*)&c3.b[i];            // uz, az at x + i
+            const float2 xmB = *(const float2*)&c3.b[-i];
+            const float2 ypA = *((const float2*)&c3.a[i * LX] + 1);  // uy, ay at y + i
+            const float2 ymA = *((const float2*)&c3.a[-i * LX] + 1);
+            const float2 ypB = *(const float2*)&c3.b[i * LX];       // uz, az at y + i
+            const float2 ymB = *(const float2*)&c3.b[-i * LX];
+            const float d0 = cx * (xpB.x - xmB.x);
+            const float d1 = cy * (ypB.x - ymB.x);
+            const float d2 = cx * (xpA.x - xmA.x) + cy * (ypA.x - ymA.x);
+            const float d3 = cx * (xpB.y - xmB.y);
+            const float d4 = cy * (ypB.y - ymB.y);
+            const float d5 = cx * (xpA.y - xmA.y) + cy * (ypA.y - ymA.y);
+            st.x[0][lo] += d0;
+            st.x[1][lo] += d1;
+            st.x[2][lo] += d2;
+            st.x[3][lo] += d3;
+            st.x[4][lo] += d4;
+            st.x[5][lo] += d5;
+            if (i == 3) { // first contribution to output p + 3
+                st.x[0][hi] = -d0;
+                st.x[1][hi] = -d1;
+                st.x[2][hi] = -d2;
+                st.x[3][hi] = -d3;
+                st.x[4][hi] = -d4;
+                st.x[5][hi] = -d5;
+            }
+            else {
+                st.x[0][hi] -= d0;
+                st.x[1][hi] -= d1;
+                st.x[2][hi] -= d2;
+                st.x[3][hi] -= d3;
+                st.x[4][hi] -= d4;
+                st.x[5][hi] -= d5;
+            }
+        }
+        const CellPtr c0 = cells((p + 1) & 3); // plane z
+        if (!OUT) {
+            push_queue(st, c0[0]);
+            return;
+        }
+
+        // ---- output plane z = p - 3 ----
+        constexpr int OS              = (PH + 4) % 7;
+        const CellPtr c1 = cells((p + 2) & 3); // plane z + 1
+        const CellPtr c2 = cells((p + 3) & 3); // plane z + 2
+        float v[NF], dx[NF], dy[NF], dxx[NF], dyy[NF], dz[NF], dzz[NF];
+        {
+            const AosCell m0 = c0[0], m1 = c1[0], m2 = c2[0], m3 = c3[0];
+#pragma unroll
+            for (int k = 0; k < NF; ++k) {
+                const float zq[7] = {st.qv[k][0], st.qv[k][1], st.qv[k][2], cell_get(m0, k),
+                                     cell_get(m1, k), cell_get(m2, k), cell_get(m3, k)};
+                v[k] = zq[3];
+                axis_queue(zq, P.d1[2], P.d2[2], dz[k], dzz[k]);
+            }
+        }
+        // in-plane first and second derivatives of all eight fields, stencil point by point
+        {
+            AosCell xp[3], xm[3], yp[3], ym[3];
+#pragma unroll
+            for (int i = 1; i <= 3; ++i) {
+                xp[i - 1] = c0[i];
+                xm[i - 1] = c0[-i];
+                yp[i - 1] = c0[i * LX];
+                ym[i - 1] = c0[-i * LX];
+            }
+#pragma unroll
+            for (int k = 0; k < NF; ++k) {
+                const float p1 = cell_get(xp[0], k), m1 = cell_get(xm[0], k);
+                const float p2 = cell_get(xp[1], k), m2 = cell_get(xm[1], k);
+                const float p3 = cell_get(xp[2], k), m3 = cell_get(xm[2], k);
+                dx[k]  = P.d1[0][1] * (p1 - m1) + P.d1[0][2] * (p2 - m2) + P.d1[0][3] * (p3 - m3);
+                dxx[k] = P.d2[0][0] * v[k] + P.d2[0][1] * (p1 + m1) + P.d2[0][2] * (p2 + m2) +
+                         P.d2[0][3] * (p3 + m3);
+                const float q1 = cell_get(yp[0], k), n1 = cell_get(ym[0], k);
+                const float q2 = cell_get(yp[1], k), n2 = cell_get(ym[1], k);
+                const float q3 = cell_get(yp[2], k), n3 = cell_get(ym[2], k);
+                dy[k]  = P.d1[1][1] * (q1 - n1) + P.d1[1][2] * (q2 - n2) + P.d1[1][3] * (q3 - n3);
+                dyy[k] = P.d2[1][0] * v[k] + P.d2[1][1] * (q1 + n1) + P.d2[1][2] * (q2 + n2) +
+                         P.d2[1][3] * (q3 + n3);
+            }
+        }
+        // x-y cross derivatives of ux, ax, uy, ay (group A)
+        float4 xy = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int i = 1; i <= 3; ++i) {
+            const int pp = i * (1 + LX), pm = i * (1 - LX);
+            const float4 a1 = c0.a[pp], a2 = c0.a[-pp], a3 = c0.a[pm], a4 = c0.a[-pm];
+            const float cc = P.cxy[i];
+            xy.x += cc * ((a1.x + a2.x) - (a3.x + a4.x));
+            xy.y += cc * ((a1.y + a2.y) - (a3.y + a4.y));
+            xy.z += cc * ((a1.z + a2.z) - (a3.z + a4.z));
+            xy.w += cc * ((a1.w + a2.w) - (a3.w + a4.w));
+        }
+        // LDS order: 0 ux, 1 ax, 2 uy, 3 ay, 4 uz, 5 az, 6 lnrho, 7 s
+        MhdAgg g;
+        g.lr_v = v[6], g.lr_x = dx[6], g.lr_y = dy[6], g.lr_z = dz[6];
+        g.lr_lap = dxx[6] + dyy[6] + dzz[6];
+        g.s_v = v[7], g.s_x = dx[7], g.s_y = dy[7], g.s_z = dz[7];
+        g.s_lap = dxx[7] + dyy[7] + dzz[7];
+        g.uv[0] = v[0], g.ug[0][0] = dx[0], g.ug[0][1] = dy[0], g.ug[0][2] = dz[0];
+        g.uv[1] = v[2], g.ug[1][0] = dx[2], g.ug[1][1] = dy[2], g.ug[1][2] = dz[2];
+        g.uv[2] = v[4], g.ug[2][0] = dx[4], g.ug[2][1] = dy[4], g.ug[2][2] = dz[4];
+        g.ulap[0] = dxx[0] + dyy[0] + dzz[0];
+        g.ulap[1] = dxx[2] + dyy[2] + dzz[2];
+        g.ulap[2] = dxx[4] + dyy[4] + dzz[4];
+        g.gdu[0] = dxx[0] + st.x[0][OS] + xy.z; // ux_xx + uz_xz + uy_xy
+        g.gdu[1] = xy.x + st.x[1][OS] + dyy[2]; // ux_xy + uz_yz + uy_yy
+        g.gdu[2] = st.x[2][OS] + dzz[4];        // ux_xz + uy_yz + uz_zz
+        g.alap[0] = dxx[1] + dyy[1] + dzz[1];
+        g.alap[1] = dxx[3] + dyy[3] + dzz[3];
+        g.alap[2] = dxx[5] + dyy[5] + dzz[5];
+        g.jj[0] = (st.x[3][OS] - (dyy[1] + dzz[1])) + xy.w; // az_xz - (ax_yy + ax_zz) + ay_xy
+        g.jj[1] = (xy.y + st.x[4][OS]) - (dxx[3] + dzz[3]);  // ax_xy + az_yz - (ay_xx + ay_zz)
+        g.jj[2] = st.x[5][OS] - (dxx[5] + dyy[5]);          // ax_xz + ay_yz - (az_xx + az_yy)
+        g.B[0] = -dz[3] + dy[5];                            // az_y - ay_z
+        g.B[1] = dz[1] - dx[5];                             // ax_z - az_x
+        g.B[2] = -dy[1] + dx[3];                            // ay_x - ax_y
+
+        float rate[NF];
+        mhd_rates_agg(g, P, rate);
+        push_queue(st, c0[0]); // qv[k][2] is now plane z
+        if (z < ze && active) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                const float cur = st.qv[AOS_POS[f]][2];
+                float outv;
+                if (a.rk.step == 0)
+                    outv = cur + a.rk.beta_dt * rate[f];
+                else
+                    outv = cur + a.rk.beta * (a.rk.ab * (cur - prev[f]) + a.rk.dt * rate[f]);
+                *(float*)((char*)(a.out[f] + oz) + ow) = outv;
+            }
+        }
+    }
+};
+
+template <int TX, int TY, int W>
+__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W)))
+solve_zaos(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
+{
+    typedef ZMarchAos<TX, TY> K;
+    __shared__ typename K::Shared sh;
+    const int b   = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int btx = b % ntx;
+    const int bty = (b / ntx) % nty;
+    const int btz = b / (ntx * nty);
+    K k(a, sh);
+    const int tid = threadIdx.x;
+    const int tx = tid % TX, ty = tid / TX;
+    const int x0 = a.start.x + btx * TX, y0 = a.start.y + bty * TY;
+    const int zs = a.start.z + btz * zdepth;
+    k.tid        = tid;
+    k.ze         = min(zs + zdepth, a.end.z);
+    k.zmax       = a.mz - 1;
+    const int gx = x0 + tx, gy = y0 + ty;
+    k.active     = gx < a.end.x && gy < a.end.y;
+    k.own        = (unsigned)(k.active ? gx + gy * a.mx : x0 + y0 * a.mx) * sizeof(float);
+#pragma unroll
+    for (int r = 0; r < K::R; ++r) {
+        const int e  = tid + r * K::NT;
+        const int ex = e % K::LX, ey = e / K::LX;
+        const int xx = x0 - 3 + ex, yy = y0 - 3 + ey;
+        k.goff[r] = (unsigned)((e < K::LN && xx < a.mx && yy < a.my) ? xx + yy * a.mx : 0) *
+                    sizeof(float);
+    }
+    k.cofs = (ty + 3) * K::LX + (tx + 3);
+
+    typename K::State st;
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            st.qv[f][j] = 0;
+#pragma unroll
+    for (int n = 0; n < 6; ++n)
+#pragma unroll
+        for (int j = 0; j < 7; ++j)
+            st.x[n][j] = 0;
+
+    const int pbeg = zs - 3;
+    k.issue_loads(st, pbeg);
+    k.template step<0, false>(st, pbeg);
+    k.template step<1, false>(st, pbeg + 1);
+    k.template step<2, false>(st, pbeg + 2);
+    k.template step<3, false>(st, pbeg + 3);
+    k.template step<4, false>(st, pbeg + 4);
+    k.template step<5, false>(st, pbeg + 5);
+    for (int p = pbeg + 6; p < k.ze + 3; p += 7) {
+        k.template step<6, true>(st, p);
+        k.template step<0, true>(st, p + 1);
+        k.template step<1, true>(st, p + 2);
+        k.template step<2, true>(st, p + 3);
+        k.template step<3, true>(st, p + 4);
+        k.template step<4, true>(st, p + 5);
+        k.template step<5, true>(st, p + 6);
+    }
+}
+#endif // AC_DOUBLE_PRECISION == 0
+
 // One z-march column: tile (btx, bty) of box `a`, output planes [zs, min(zs + zdepth, end.z)).
 template <int TX, int TY>
 __device__ __forceinline__ void
@@ -950,6 +1302,11 @@ constexpr ConfigEntry kConfigs[] = {
     {3, 32, 4},
     {3, 16, 8},
     {4, 16, 16},
+    {5, 32, 4},
+    {5, 16, 8},
+    {5, 16, 16},
+    {6, 16, 16},
+    {6, 32, 4},
 #endif
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
@@ -1083,6 +1440,16 @@ launch_substep(const AcKernelConfig& c, const SubstepArgs& a, hipStream_t stream
         return launch_zmarch_grid<16, 8>(solve_zlean<16, 8, 1>, a, c.z_depth, stream);
     if (c.variant == 4 && c.tile_x == 16 && c.tile_y == 16)
         return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2>, a, c.z_depth, stream);
+    if (c.variant == 5 && c.tile_x == 32 && c.tile_y == 4)
+        return launch_zmarch_grid<32, 4>(solve_zaos<32, 4, 1>, a, c.z_depth, stream);
+    if (c.variant == 5 && c.tile_x == 16 && c.tile_y == 8)
+        return launch_zmarch_grid<16, 8>(solve_zaos<16, 8, 1>, a, c.z_depth, stream);
+    if (c.variant == 5 && c.tile_x == 16 && c.tile_y == 16)
+        return launch_zmarch_grid<16, 16>(solve_zaos<16, 16, 1>, a, c.z_depth, stream);
+    if (c.variant == 6 && c.tile_x == 16 && c.tile_y == 16)
+        return launch_zmarch_grid<16, 16>(solve_zaos<16, 16, 2>, a, c.z_depth, stream);
+    if (c.variant == 6 && c.tile_x == 32 && c.tile_y == 4)
+        return launch_zmarch_grid<32, 4>(solve_zaos<32, 4, 2>, a, c.z_depth, stream);
 #endif
 #undef AC_DIRECT
 #undef AC_ZMARCH
