@@ -8,8 +8,9 @@
  * against the reference Device interface compile and link against this header and
  * libastaroth_fp32.so / libastaroth_fp64.so unchanged (AC_DOUBLE_PRECISION selects which).
  *
- * The reference's Node, Grid and legacy ac* layers are multi-GPU wrappers over this
- * interface and are not declared here.
+ * The reference's Node and Grid layers are multi-GPU wrappers over this interface and are
+ * not declared here. Its legacy single-call interface (acInit ... acQuit, :208-283) is
+ * provided over one Device (one GPU); acGetNode is not, since there is no Node.
  */
 #pragma once
 #include <float.h>  /* FLT_EPSILON, etc */
@@ -283,6 +284,38 @@ AcResult acHostMeshCreate(const AcMeshInfo mesh_info, AcMesh* mesh);
  * from the C library's rand(), whose sequence is platform specific). */
 AcResult acHostMeshRandomize(AcMesh* mesh);
 AcResult acHostMeshDestroy(AcMesh* mesh);
+
+/* ---- Legacy interface (ref :208-283, src/core/astaroth.cc:28-188) over one Device ----
+ * acIntegrate is 3 x {periodic fill, substep, swap} and synchronises; acReduce* finalise the
+ * RMS types as sqrt(sum / n_x n_y n_z), as the reference's Node does (node.cc:932-957);
+ * acLoad is asynchronous on STREAM_DEFAULT, acStore synchronous. The *GBC variants (general
+ * boundaries) return AC_FAILURE. */
+AcResult acInit(const AcMeshInfo mesh_info);
+AcResult acQuit(void);
+AcResult acCheckDeviceAvailability(void);
+AcResult acSynchronizeStream(const Stream stream);
+AcResult acSynchronizeMesh(void);
+AcResult acLoadDeviceConstant(const AcRealParam param, const AcReal value);
+AcResult acLoad(const AcMesh host_mesh);
+AcResult acSetVertexBuffer(const VertexBufferHandle handle, const AcReal value);
+AcResult acStore(AcMesh* host_mesh);
+AcResult acIntegrate(const AcReal dt);
+AcResult acIntegrateGBC(const AcMeshInfo config, const AcReal dt);
+AcResult acBoundcondStep(void);
+AcResult acBoundcondStepGBC(const AcMeshInfo config);
+AcReal acReduceScal(const ReductionType rtype, const VertexBufferHandle vtxbuf_handle);
+AcReal acReduceVec(const ReductionType rtype, const VertexBufferHandle a,
+                   const VertexBufferHandle b, const VertexBufferHandle c);
+AcReal acReduceVecScal(const ReductionType rtype, const VertexBufferHandle a,
+                       const VertexBufferHandle b, const VertexBufferHandle c,
+                       const VertexBufferHandle d);
+AcResult acStoreWithOffset(const int3 dst, const size_t num_vertices, AcMesh* host_mesh);
+AcResult acIntegrateStep(const int isubstep, const AcReal dt);
+AcResult acIntegrateStepWithOffset(const int isubstep, const AcReal dt, const int3 start,
+                                   const int3 end);
+AcResult acSynchronize(void);
+AcResult acLoadWithOffset(const AcMesh host_mesh, const int3 src, const int num_vertices);
+int acGetNumDevicesPerNode(void);
 
 #ifdef __cplusplus
 } /* extern "C" */
