@@ -140,6 +140,7 @@ class Library:
             "acEnsembleDiagnostics": (i, [vp, ctypes.POINTER(ctypes.c_double)]),
             "acEnsembleSynchronize": (i, [vp]),
             "acEnsembleGetStreamHandle": (vp, [vp]),
+            "acDeviceSetCompensatedIntegration": (i, [vp, i]),
             "acDevicePackData": (i, [vp, i, I3, I3, vp]),
             "acDeviceUnpackData": (i, [vp, i, vp, I3, I3]),
             "acDeviceHaloPackedSize": (ctypes.c_size_t, [vp]),
@@ -372,6 +373,10 @@ class Device:
     def randomize_uniform(self, handle, lo, hi, seed, stream=abi.STREAM_DEFAULT):
         return self._check(self.lib.dll.acDeviceRandomizeUniform(self.handle, stream, _vh(handle), lo, hi, seed),
                            "acDeviceRandomizeUniform")
+
+    def set_compensated_integration(self, enable=True):
+        return self._check(self.lib.dll.acDeviceSetCompensatedIntegration(self.handle, int(bool(enable))),
+                           "acDeviceSetCompensatedIntegration")
 
     def randomize_uniform_sinz(self, handle, lo, hi, seed, amplitude, phase_per_cell, stream=abi.STREAM_DEFAULT):
         return self._check(self.lib.dll.acDeviceRandomizeUniformSinZ(self.handle, stream, _vh(handle), lo, hi,

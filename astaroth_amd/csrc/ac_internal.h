@@ -68,6 +68,10 @@ struct RK3Coeffs {
 struct SubstepArgs {
     const AcReal* __restrict__ in[NF];
     AcReal* __restrict__ out[NF];
+    // Compensated integration only (null otherwise): rounding-error terms of `in` (read) and of
+    // `out` (read as the previous stage's, then overwritten with the new ones).
+    const AcReal* __restrict__ cin[NF];
+    AcReal* __restrict__ cout[NF];
     int mx, my, mz, mxy;
     int3 start, end;
     PhysParams phys;
@@ -137,6 +141,9 @@ struct device_s {
     double* reduce_result;
     double* reduce_result_host; // pinned
     AcKernelConfig kernel_config;
+    // compensated integration (acDeviceSetCompensatedIntegration): comp[0] pairs with vba.in,
+    // comp[1] with vba.out; null when off
+    AcReal* comp[2][ac::NF];
     // acDeviceIntegrateStepGraph cache: (stream, dt bits) -> (graph exec, which in/out parity
     // the graph was recorded for)
     std::map<std::pair<int, unsigned long long>, hipGraphExec_t> graphs[2];

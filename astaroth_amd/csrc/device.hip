@@ -86,12 +86,15 @@ tune_cache()
 
 SubstepArgs
 make_args(const Device d, const AcReal* const* in, AcReal* const* out, int step, int3 start,
-          int3 end, AcReal dt)
+          int3 end, AcReal dt, int par = 0)
 {
+    // `par` = 1 when `in` is the device's current `out` set (inside a recorded step)
     SubstepArgs a;
     for (int q = 0; q < NF; ++q) {
-        a.in[q]  = in[q];
-        a.out[q] = out[q];
+        a.in[q]   = in[q];
+        a.out[q]  = out[q];
+        a.cin[q]  = d->comp[par][q];
+        a.cout[q] = d->comp[1 - par][q];
     }
     a.mx    = nx_of(d) + 6;
     a.my    = ny_of(d) + 6;
@@ -448,6 +451,8 @@ acDeviceCreate(const int id, const AcMeshInfo config, Device* device_handle)
     d->parity       = 0;
     d->halo_buffer     = nullptr;
     d->halo_buffer_len = 0;
+    for (int q = 0; q < NF; ++q)
+        d->comp[0][q] = d->comp[1][q] = nullptr;
     for (int i = 0; i < NUM_STREAMS; ++i)
         AC_HIP_CHECK(hipStreamCreateWithFlags(&d->streams[i], hipStreamNonBlocking));
 
@@ -491,6 +496,10 @@ acDeviceDestroy(Device device)
     AC_HIP_CHECK(hipHostFree(device->reduce_result_host));
     if (device->halo_buffer)
         AC_HIP_CHECK(hipFree(device->halo_buffer));
+    for (int set = 0; set < 2; ++set)
+        for (int q = 0; q < NF; ++q)
+            if (device->comp[set][q])
+                AC_HIP_CHECK(hipFree(device->comp[set][q]));
     for (int i = 0; i < NUM_STREAMS; ++i)
         AC_HIP_CHECK(hipStreamDestroy(device->streams[i]));
     delete device;
@@ -500,10 +509,28 @@ acDeviceDestroy(Device device)
 AcResult
 acDeviceSwapBuffers(const Device device)
 {
-    for (int q = 0; q < NF; ++q)
+    for (int q = 0; q < NF; ++q) {
         std::swap(device->vba.in[q], device->vba.out[q]);
+        std::swap(device->comp[0][q], device->comp[1][q]);
+    }
     device->parity ^= 1;
     return AC_SUCCESS;
+}
+
+// Values written into `in` from outside the integrator are exact: their compensation terms
+// (compensated integration only) restart at zero.
+static hipError_t
+clear_compensation(const Device d, int handle, hipStream_t s)
+{
+    if (!d->comp[0][handle])
+        return hipSuccess;
+    const long n = (long)acVertexBufferSize(d->local_config);
+    for (int set = 0; set < 2; ++set) {
+        const hipError_t e = launch_fill_constant(d->comp[set][handle], n, AcReal(0), s);
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
 }
 
 AcResult
@@ -522,6 +549,7 @@ acDeviceLoadVertexBufferWithOffset(const Device device, const Stream stream, con
                                 host_mesh.vertex_buffer[vtxbuf_handle] + src_idx,
                                 sizeof(AcReal) * (size_t)num_vertices, hipMemcpyHostToDevice,
                                 device->streams[stream]));
+    AC_HIP_CHECK(clear_compensation(device, vtxbuf_handle, device->streams[stream]));
     return AC_SUCCESS;
 }
 
@@ -564,6 +592,7 @@ acDeviceSetVertexBuffer(const Device device, const Stream stream, const VertexBu
     const long n = (long)acVertexBufferSize(device->local_config);
     AC_HIP_CHECK(launch_fill_constant(device->vba.in[handle], n, value, device->streams[stream]));
     AC_HIP_CHECK(launch_fill_constant(device->vba.out[handle], n, value, device->streams[stream]));
+    AC_HIP_CHECK(clear_compensation(device, handle, device->streams[stream]));
     return AC_SUCCESS;
 }
 
@@ -968,7 +997,7 @@ acDeviceIntegrateStepGraph(const Device device, const Stream stream, const AcRea
             AcReal* const* cin  = bufs[st & 1];
             AcReal* const* cout = bufs[(st + 1) & 1];
             AC_HIP_CHECK(launch_periodic_fill(cin, NF, nx, ny, nz, zero, m, s));
-            const SubstepArgs a = make_args(device, cin, cout, st, nmin, nmax, dt);
+            const SubstepArgs a = make_args(device, cin, cout, st, nmin, nmax, dt, st & 1);
             AC_HIP_CHECK(launch_substep(device->kernel_config, a, s));
         }
         hipGraph_t graph;
@@ -1082,6 +1111,7 @@ acDeviceRandomizeUniform(const Device device, const Stream stream, const VertexB
     AC_HIP_CHECK(hipSetDevice(device->id));
     AC_HIP_CHECK(launch_fill_uniform(device->vba.in[handle], device->local_config, lo, hi, seed,
                                      device->streams[stream]));
+    AC_HIP_CHECK(clear_compensation(device, handle, device->streams[stream]));
     return AC_SUCCESS;
 }
 
@@ -1097,6 +1127,34 @@ acDeviceRandomizeUniformSinZ(const Device device, const Stream stream,
     AC_HIP_CHECK(hipSetDevice(device->id));
     AC_HIP_CHECK(launch_fill_uniform(device->vba.in[handle], device->local_config, lo, hi, seed,
                                      device->streams[stream], amplitude, phase_per_cell));
+    AC_HIP_CHECK(clear_compensation(device, handle, device->streams[stream]));
+    return AC_SUCCESS;
+}
+
+AcResult
+acDeviceSetCompensatedIntegration(const Device device, const int enable)
+{
+    if (!device)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(device->id));
+    AC_HIP_CHECK(hipDeviceSynchronize());
+    const bool on = device->comp[0][0] != nullptr;
+    if (enable && !on) {
+        const size_t bytes = acVertexBufferSizeBytes(device->local_config);
+        for (int set = 0; set < 2; ++set)
+            for (int q = 0; q < NF; ++q) {
+                AC_HIP_CHECK(hipMalloc((void**)&device->comp[set][q], bytes));
+                AC_HIP_CHECK(hipMemset(device->comp[set][q], 0, bytes));
+            }
+    }
+    else if (!enable && on) {
+        for (int set = 0; set < 2; ++set)
+            for (int q = 0; q < NF; ++q) {
+                AC_HIP_CHECK(hipFree(device->comp[set][q]));
+                device->comp[set][q] = nullptr;
+            }
+    }
+    invalidate_graphs(device);
     return AC_SUCCESS;
 }
 

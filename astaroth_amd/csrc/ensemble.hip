@@ -262,8 +262,10 @@ build_args(Ensemble e, AcReal dt)
                 const int3 n    = dims_of(e->info[b]);
                 SubstepArgs& x  = a[b];
                 for (int q = 0; q < NF; ++q) {
-                    x.in[q]  = e->fields[par][q][b];
-                    x.out[q] = e->fields[1 - par][q][b];
+                    x.in[q]   = e->fields[par][q][b];
+                    x.out[q]  = e->fields[1 - par][q][b];
+                    x.cin[q]  = nullptr;
+                    x.cout[q] = nullptr;
                 }
                 x.mx    = n.x + 6;
                 x.my    = n.y + 6;
@@ -628,8 +630,10 @@ acEnsembleIntegrateStepLooped(Ensemble e, const AcReal dt)
             AC_HIP_CHECK(launch_periodic_fill(cin, NF, n.x, n.y, n.z, zero, m, e->stream));
             SubstepArgs a;
             for (int q = 0; q < NF; ++q) {
-                a.in[q]  = cin[q];
-                a.out[q] = cout[q];
+                a.in[q]   = cin[q];
+                a.out[q]  = cout[q];
+                a.cin[q]  = nullptr;
+                a.cout[q] = nullptr;
             }
             a.mx    = n.x + 6;
             a.my    = n.y + 6;

@@ -604,7 +604,7 @@ solve_zmarch2(const SubstepArgs a, const int zdepth, const int ntx, const int nt
 // * Global loads and stores use a uniform per-plane base and a 32-bit per-thread byte offset.
 // * The right-hand side is formed field by field into the aggregates of MhdAgg, which keeps the
 //   live set small.
-template <int TX, int TY> struct ZMarchLean {
+template <int TX, int TY, bool COMP = false> struct ZMarchLean {
     static constexpr int NT  = TX * TY;
     static constexpr int LX  = TX + 6, LY = TY + 6, LN = LX * LY;
     static constexpr int R   = (LN + NT - 1) / NT;
@@ -815,6 +815,10 @@ template <int TX, int TY> struct ZMarchLean {
         AcReal rate[NF];
         mhd_rates_agg(g, P, rate);
         push_queue(st, ro); // qv[f][2] is now plane z
+        if (COMP) {
+            store_compensated(st, rate, prev, oz, ow, z);
+            return;
+        }
         if (z < ze && active) {
             if (a.rk.step == 0) {
 #pragma unroll
@@ -829,6 +833,40 @@ template <int TX, int TY> struct ZMarchLean {
                         cur + a.rk.beta * (a.rk.ab * (cur - prev[f]) + a.rk.dt * rate[f]);
                 }
             }
+        }
+    }
+
+    // Compensated update: the state is the pair (u, c) with u + c carrying the rounding error
+    // of every previous update (u in `in`/`out`, c in `cin`/`cout`). The increment is formed
+    // from the pairs and added with an error-free two-sum (Knuth's TwoSum), whose error term
+    // becomes the new compensation:
+    //   s = 0: d = beta_dt F + c_in
+    //   s > 0: d = beta (ab ((u_in - u_prev) + (c_in - c_prev)) + dt F) + c_in
+    //   u_out + c_out = u_in + d exactly.
+    // The stencils read u only.
+    __device__ __forceinline__ void store_compensated(const State& st, const AcReal* rate,
+                                                      const AcReal* prev, unsigned long oz,
+                                                      unsigned ow, int z) const
+    {
+        if (!(z < ze && active))
+            return;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const AcReal u  = st.qv[f][2];
+            const AcReal ci = ld(a.cin[f] + oz, ow);
+            AcReal d;
+            if (a.rk.step == 0) {
+                d = a.rk.beta_dt * rate[f] + ci;
+            }
+            else {
+                const AcReal cp = ld(a.cout[f] + oz, ow);
+                d = a.rk.beta * (a.rk.ab * ((u - prev[f]) + (ci - cp)) + a.rk.dt * rate[f]) + ci;
+            }
+            const AcReal sum = u + d;
+            const AcReal bb  = sum - u;
+            const AcReal err = (u - (sum - bb)) + (d - bb);
+            *(AcReal*)((char*)(a.out[f] + oz) + ow)  = sum;
+            *(AcReal*)((char*)(a.cout[f] + oz) + ow) = err;
         }
     }
 };
@@ -1186,12 +1224,12 @@ solve_zaos(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
 #endif // AC_DOUBLE_PRECISION == 0
 
 // One z-march column: tile (btx, bty) of box `a`, output planes [zs, min(zs + zdepth, end.z)).
-template <int TX, int TY>
+template <int TX, int TY, bool COMP = false>
 __device__ __forceinline__ void
-zlean_column(const SubstepArgs& a, typename ZMarchLean<TX, TY>::Shared& sh, const int btx,
+zlean_column(const SubstepArgs& a, typename ZMarchLean<TX, TY, COMP>::Shared& sh, const int btx,
              const int bty, const int zs, const int zdepth)
 {
-    typedef ZMarchLean<TX, TY> K;
+    typedef ZMarchLean<TX, TY, COMP> K;
     K k(a, sh);
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
@@ -1245,16 +1283,16 @@ zlean_column(const SubstepArgs& a, typename ZMarchLean<TX, TY>::Shared& sh, cons
     }
 }
 
-template <int TX, int TY, int W>
+template <int TX, int TY, int W, bool COMP = false>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W)))
 solve_zlean(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
 {
-    __shared__ typename ZMarchLean<TX, TY>::Shared sh;
+    __shared__ typename ZMarchLean<TX, TY, COMP>::Shared sh;
     const int b   = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int btx = b % ntx;
     const int bty = (b / ntx) % nty;
     const int btz = b / (ntx * nty);
-    zlean_column<TX, TY>(a, sh, btx, bty, a.start.z + btz * zdepth, zdepth);
+    zlean_column<TX, TY, COMP>(a, sh, btx, bty, a.start.z + btz * zdepth, zdepth);
 }
 
 // Persistent form for ensembles of independent boxes: a grid sized to the resident block slots
@@ -1405,6 +1443,15 @@ launch_substep(const AcKernelConfig& c, const SubstepArgs& a, hipStream_t stream
 {
     if (a.end.x <= a.start.x || a.end.y <= a.start.y || a.end.z <= a.start.z)
         return hipSuccess;
+    if (a.cin[0] != nullptr) {
+        // compensated integration: the lean z-march in its default tile, z-depth from `c`
+        const int zd = c.variant >= 1 ? c.z_depth : 0;
+#if AC_DOUBLE_PRECISION == 0
+        return launch_zmarch_grid<32, 4>(solve_zlean<32, 4, 1, true>, a, zd, stream);
+#else
+        return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 1, true>, a, zd, stream);
+#endif
+    }
 #define AC_DIRECT(TX, TY)                                                                          \
     if (c.variant == 0 && c.tile_x == TX && c.tile_y == TY)                                        \
         return launch_direct<TX, TY>(a, stream);

@@ -156,11 +156,13 @@ def precision_study(n: int, steps: int, dt: float, hip: Hip):
     from astaroth_amd.abi import VTXBUF_NAMES
 
     states, diag, times = {}, {}, {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "fp32_compensated"):
         lib = Library(prec == "fp64")
         info = lib.config(n=n)
         dev = lib.device_create(0, info, autotune=False)
         try:
+            if prec == "fp32_compensated":
+                dev.set_compensated_integration(True)
             device_init_mhd(dev, config_id=3, perturb_sin2z=True)
             diag[prec] = []
             hip.sync()
@@ -176,25 +178,30 @@ def precision_study(n: int, steps: int, dt: float, hip: Hip):
             states[prec] = [f[3:-3, 3:-3, 3:-3] for f in mesh.fields]
         finally:
             dev.destroy()
-    fields = {}
-    worst = 0.0
-    for q, name in enumerate(VTXBUF_NAMES):
-        ref = states["fp64"][q]
-        d = np.abs(states["fp32"][q].astype(np.float64) - ref)
-        i = int(np.argmax(d))
-        rng = float(ref.max() - ref.min())
-        rn = float(d.flat[i]) / rng if rng > 0 else float(d.flat[i])
-        at = float(d.flat[i]) / abs(float(ref.flat[i])) if ref.flat[i] != 0 else float("inf")
-        fields[name] = {"max_abs": float(d.flat[i]), "range": rng, "range_normalised": rn,
-                        "rel_at_max_abs_point": at}
-        worst = max(worst, rn)
-        del d
-    ddiag = {}
-    for key in ("uu", "aa", "lnrho"):
-        last64, last32 = diag["fp64"][-1][key], diag["fp32"][-1][key]
-        ddiag[key] = [abs(a - b) / abs(a) if a != 0 else abs(b) for a, b in zip(last64, last32)]
-    return dict(fields=fields, worst_range_normalised=worst, target=1e-5, passed=worst <= 1e-5,
-                diagnostics_rel_diff_last_step=ddiag, s_per_step=times)
+    out = {}
+    for mode in ("fp32", "fp32_compensated"):
+        fields = {}
+        worst = 0.0
+        for q, name in enumerate(VTXBUF_NAMES):
+            ref = states["fp64"][q]
+            d = np.abs(states[mode][q].astype(np.float64) - ref)
+            i = int(np.argmax(d))
+            rng = float(ref.max() - ref.min())
+            rn = float(d.flat[i]) / rng if rng > 0 else float(d.flat[i])
+            at = float(d.flat[i]) / abs(float(ref.flat[i])) if ref.flat[i] != 0 else float("inf")
+            fields[name] = {"max_abs": float(d.flat[i]), "range": rng, "range_normalised": rn,
+                            "rel_at_max_abs_point": at}
+            worst = max(worst, rn)
+            del d
+        ddiag = {}
+        for key in ("uu", "aa", "lnrho"):
+            last64, last32 = diag["fp64"][-1][key], diag[mode][-1][key]
+            ddiag[key] = [abs(a - b) / abs(a) if a != 0 else abs(b) for a, b in zip(last64, last32)]
+        out[mode] = dict(fields=fields, worst_range_normalised=worst, passed=worst <= 1e-5,
+                         diagnostics_rel_diff_last_step=ddiag)
+    r = dict(out["fp32"])
+    r.update(target=1e-5, s_per_step=times, compensated=out["fp32_compensated"])
+    return r
 
 
 def copy_probe(double: bool, n: int, reps: int, hip: Hip):

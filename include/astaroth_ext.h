@@ -59,6 +59,15 @@ AcResult acDeviceSetKernelConfig(const Device device, const AcKernelConfig confi
  * the device's current buffers. Returns a negative value for an unusable configuration. */
 float acDeviceTimeKernelConfig(const Device device, const AcKernelConfig config, const int iters);
 
+/* ---- Compensated integration (the precision study's FP32 mode) ----
+ * With enable != 0 the Device keeps, for every field, the rounding error of its stored state in
+ * a second buffer (two more buffer sets, allocated and zeroed here) and each substep adds its
+ * increment with an error-free two-sum: u_out + c_out = u_in + d exactly, where d includes
+ * c_in. The right-hand side still reads the rounded state only. Loads, sets and randomisations
+ * of a field reset its compensation to zero. The substep then always uses the lean z-march in
+ * its default tile. enable == 0 frees the buffers. */
+AcResult acDeviceSetCompensatedIntegration(const Device device, const int enable);
+
 /* ---- Synthetic initial conditions ----
  * Fills the interior of `in[handle]` with lo + (hi - lo) u, u in [0, 1) from a counter-based
  * splitmix64 stream keyed by `seed` and the linear interior cell index (asynchronous on
