@@ -141,6 +141,8 @@ class Library:
             "acEnsembleSynchronize": (i, [vp]),
             "acEnsembleGetStreamHandle": (vp, [vp]),
             "acDeviceSetCompensatedIntegration": (i, [vp, i]),
+            "acDeviceTimestepCFL": (i, [vp, i, ctypes.POINTER(ctypes.c_double)]),
+            "acEnsembleTimestepCFL": (i, [vp, ctypes.POINTER(ctypes.c_double)]),
             "acDevicePackData": (i, [vp, i, I3, I3, vp]),
             "acDeviceUnpackData": (i, [vp, i, vp, I3, I3]),
             "acDeviceHaloPackedSize": (ctypes.c_size_t, [vp]),
@@ -374,6 +376,11 @@ class Device:
         return self._check(self.lib.dll.acDeviceRandomizeUniform(self.handle, stream, _vh(handle), lo, hi, seed),
                            "acDeviceRandomizeUniform")
 
+    def timestep_cfl(self, stream=abi.STREAM_DEFAULT) -> float:
+        out = ctypes.c_double()
+        self._check(self.lib.dll.acDeviceTimestepCFL(self.handle, stream, ctypes.byref(out)), "acDeviceTimestepCFL")
+        return out.value
+
     def set_compensated_integration(self, enable=True):
         return self._check(self.lib.dll.acDeviceSetCompensatedIntegration(self.handle, int(bool(enable))),
                            "acDeviceSetCompensatedIntegration")
@@ -506,6 +513,12 @@ class Ensemble:
 
     def synchronize(self):
         return self._check(self.lib.dll.acEnsembleSynchronize(self.handle), "acEnsembleSynchronize")
+
+    def timestep_cfl(self) -> np.ndarray:
+        out = np.zeros(self.nboxes, dtype=np.float64)
+        self._check(self.lib.dll.acEnsembleTimestepCFL(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                    "acEnsembleTimestepCFL")
+        return out
 
     def stream_handle(self):
         return self.lib.dll.acEnsembleGetStreamHandle(self.handle)

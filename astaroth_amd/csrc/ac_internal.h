@@ -126,6 +126,9 @@ hipError_t launch_pack(bool pack, AcReal* const* fields, int mx, int my, AcReal*
 // The 26 ghost blocks of an n-cell box (8 corners, 12 edges, 6 sides) with their periodic
 // interior images; returns the block count.
 int halo_blocks(int3 n, int3* ghost_origin, int3* interior_origin, int3* dims);
+// CFL time step of the reference's standalone driver from the maximum speed and Alfven speed
+// (host.cc).
+double cfl_timestep(double umax, double vamax, const AcMeshInfo& info);
 // Plain element copy (one read and one write per element).
 hipError_t launch_copy(AcReal* dst, const AcReal* src, long n, hipStream_t stream);
 

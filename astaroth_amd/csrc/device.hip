@@ -1132,6 +1132,19 @@ acDeviceRandomizeUniformSinZ(const Device device, const Stream stream,
 }
 
 AcResult
+acDeviceTimestepCFL(const Device device, const Stream stream, double* dt)
+{
+    if (!dt)
+        return AC_FAILURE;
+    double umax = 0;
+    if (acDeviceReduceVecDouble(device, stream, RTYPE_MAX, VTXBUF_UUX, VTXBUF_UUY, VTXBUF_UUZ,
+                                &umax) != AC_SUCCESS)
+        return AC_FAILURE;
+    *dt = cfl_timestep(umax, 0.0, uniforms_of(device));
+    return AC_SUCCESS;
+}
+
+AcResult
 acDeviceSetCompensatedIntegration(const Device device, const int enable)
 {
     if (!device)

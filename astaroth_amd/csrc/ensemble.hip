@@ -679,6 +679,19 @@ acEnsembleDiagnostics(Ensemble e, double* results)
 }
 
 AcResult
+acEnsembleTimestepCFL(Ensemble e, double* dts)
+{
+    if (!e || !dts)
+        return AC_FAILURE;
+    std::vector<double> d((size_t)e->nboxes * NDIAG);
+    if (acEnsembleDiagnostics(e, d.data()) != AC_SUCCESS)
+        return AC_FAILURE;
+    for (int b = 0; b < e->nboxes; ++b)
+        dts[b] = cfl_timestep(d[(size_t)b * NDIAG], 0.0, e->info[b]);
+    return AC_SUCCESS;
+}
+
+AcResult
 acEnsembleSynchronize(const Ensemble e)
 {
     if (!e)

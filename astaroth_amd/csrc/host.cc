@@ -130,3 +130,38 @@ acLoadConfig(const char* config_path, AcMeshInfo* config)
 }
 
 } // extern "C"
+
+// CFL time step of the reference's standalone driver (samples/standalone/model/host_timestep.cc:
+// 34-56, evaluated in long double as there):
+//   dt = min(cdt dsmin / (|u_max| + sqrt(cs^2 + v_A^2)),
+//            cdtv dsmin^2 / max(max(nu, eta), max(gamma, chi)))   with chi = 0,
+// dsmin = AC_dsmin, or min(dsx, dsy, dsz) when AC_dsmin is unset (the standalone configuration
+// loader sets it so, config_loader.cc:102).
+namespace ac {
+double
+cfl_timestep(double umax, double vamax, const AcMeshInfo& info)
+{
+    typedef long double L;
+    const L cdt   = info.real_params[AC_cdt];
+    const L cdtv  = info.real_params[AC_cdtv];
+    const L cs2   = info.real_params[AC_cs2_sound];
+    const L nu    = info.real_params[AC_nu_visc];
+    const L eta   = info.real_params[AC_eta];
+    const L gamma = info.real_params[AC_gamma];
+    const L chi   = 0;
+    L dsmin       = info.real_params[AC_dsmin];
+    if (!(dsmin > 0)) {
+        dsmin = info.real_params[AC_dsx];
+        if ((L)info.real_params[AC_dsy] < dsmin)
+            dsmin = info.real_params[AC_dsy];
+        if ((L)info.real_params[AC_dsz] < dsmin)
+            dsmin = info.real_params[AC_dsz];
+    }
+    const L va     = vamax;
+    const L uu_dt  = cdt * dsmin / (fabsl((L)umax) + sqrtl(cs2 + va * va));
+    const L m1     = nu > eta ? nu : eta;
+    const L m2     = gamma > chi ? gamma : chi;
+    const L visc_dt = cdtv * dsmin * dsmin / (m1 > m2 ? m1 : m2);
+    return (double)(uu_dt < visc_dt ? uu_dt : visc_dt);
+}
+} // namespace ac

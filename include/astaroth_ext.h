@@ -59,6 +59,12 @@ AcResult acDeviceSetKernelConfig(const Device device, const AcKernelConfig confi
  * the device's current buffers. Returns a negative value for an unusable configuration. */
 float acDeviceTimeKernelConfig(const Device device, const AcKernelConfig config, const int iters);
 
+/* ---- CFL time step ----
+ * dt of the reference's standalone driver (samples/standalone/model/host_timestep.cc:34-56):
+ * min(cdt dsmin / (|u|_max + cs), cdtv dsmin^2 / max(nu, eta, gamma)), with |u|_max from an FP64
+ * device reduction (synchronous) and dsmin = AC_dsmin or, when unset, min(dsx, dsy, dsz). */
+AcResult acDeviceTimestepCFL(const Device device, const Stream stream, double* dt);
+
 /* ---- Compensated integration (the precision study's FP32 mode) ----
  * With enable != 0 the Device keeps, for every field, the rounding error of its stored state in
  * a second buffer (two more buffer sets, allocated and zeroed here) and each substep adds its
@@ -148,6 +154,8 @@ AcResult acEnsembleIntegrateStepLooped(Ensemble ensemble, const AcReal dt);
 /* Diagnostics of the last step, 7 per box: max |u|, rms |u|, max |A|, rms |A|, max lnrho,
  * min lnrho, rms lnrho (FP64, over the interior; synchronous). */
 AcResult acEnsembleDiagnostics(Ensemble ensemble, double* results);
+/* The CFL time step of every box from its last-step max |u| (see acDeviceTimestepCFL). */
+AcResult acEnsembleTimestepCFL(Ensemble ensemble, double* dts);
 AcResult acEnsembleSynchronize(const Ensemble ensemble);
 void* acEnsembleGetStreamHandle(const Ensemble ensemble);
 

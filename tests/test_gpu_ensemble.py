@@ -129,3 +129,23 @@ def test_ensemble_rejects_bad_arguments(lib):
         assert d.acEnsembleIntegrateStep(ens.handle, lib.real(float("nan"))) != 0
         with pytest.raises(Exception):
             ens.set_kernel_config(64, 4, 0)
+
+
+def test_ensemble_cfl_timestep(lib):
+    """Per-box CFL dt from the last step's diagnostics: the single-box formula with each box's
+    own spacing and viscosity."""
+    import math
+    infos = box_infos(lib)
+    with lib.ensemble_create(infos) as ens:
+        for b in range(len(infos)):
+            ensemble_init_mhd(ens, b, config_id=4, box_id=b)
+        ens.periodic_boundconds()
+        ens.integrate_step(1e-3)
+        d = ens.diagnostics()
+        dts = ens.timestep_cfl()
+    for b, info in enumerate(infos):
+        rp = lambda k: float(info.real_params[RP[k]])
+        dsmin = min(rp("AC_dsx"), rp("AC_dsy"), rp("AC_dsz"))
+        uu_dt = rp("AC_cdt") * dsmin / (d[b, 0] + math.sqrt(rp("AC_cs2_sound")))
+        visc_dt = rp("AC_cdtv") * dsmin ** 2 / max(max(rp("AC_nu_visc"), rp("AC_eta")), rp("AC_gamma"))
+        assert dts[b] == pytest.approx(min(uu_dt, visc_dt), rel=1e-12)

@@ -458,3 +458,22 @@ def test_autotune_selects_a_valid_config():
     for q in range(8):
         scale = np.max(np.abs(ref[q]))
         assert np.max(np.abs(out.fields[q].astype(np.float64) - ref[q])) / scale < 1e-5
+
+
+def test_cfl_timestep(lib):
+    """acDeviceTimestepCFL against the formula of the reference's standalone driver
+    (host_timestep.cc:34-56) evaluated on the host from the stored state."""
+    n = 20
+    info = lib.config(n=n, ny=24, nz=16)
+    fields = mhd_fields(n, 24, 16, np.float64, config_id=1)
+    with make_device(lib, info) as dev:
+        dev.load_mesh(host_mesh_from(lib, info, fields))
+        dt = dev.timestep_cfl()
+    u = [fields[VTX[k]][3:-3, 3:-3, 3:-3].astype(lib.dtype).astype(np.float64)
+         for k in ("VTXBUF_UUX", "VTXBUF_UUY", "VTXBUF_UUZ")]
+    umax = np.sqrt(np.max(u[0] ** 2 + u[1] ** 2 + u[2] ** 2))
+    rp = lambda k: float(info.real_params[abi.RP[k]])
+    dsmin = min(rp("AC_dsx"), rp("AC_dsy"), rp("AC_dsz"))
+    uu_dt = rp("AC_cdt") * dsmin / (umax + math.sqrt(rp("AC_cs2_sound")))
+    visc_dt = rp("AC_cdtv") * dsmin ** 2 / max(max(rp("AC_nu_visc"), rp("AC_eta")), rp("AC_gamma"))
+    assert dt == pytest.approx(min(uu_dt, visc_dt), rel=1e-12)
