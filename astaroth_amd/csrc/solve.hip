@@ -24,11 +24,16 @@
 //  variant 2: variant 1 with the plane loop unrolled by the 7-plane queue length, so queue and
 //      accumulator slots are compile-time registers and no register moves rotate them.
 //
-//  variants 3, 4 ("lean z-march", the default): all eight fields in one 4-slot LDS ring, a
-//      3-register z queue per field, six combined cross-derivative sums instead of eight, a
-//      branch-free plane loop and a right-hand side formed field by field. About 25 % fewer
-//      issued instructions per cell than variant 2; variant 4 is the same kernel compiled for
-//      two waves per SIMD.
+//  variants 3, 4 ("lean z-march"): all eight fields in one 4-slot LDS ring, a 3-register z
+//      queue per field, six combined cross-derivative sums instead of eight, a branch-free plane
+//      loop and a right-hand side formed field by field. About 25 % fewer issued instructions
+//      per cell than variant 2; variant 4 is the same kernel compiled for two waves per SIMD.
+//  variants 7-12: the lean z-march with a 5-slot ring (7: one barrier per plane), or loading
+//      the previous stage before the next plane and storing each output plane at the start of
+//      the next step (8: 4 slots, the FP64 default; 9: 5 slots), and its two-waves-per-SIMD
+//      builds on 16 x 16 tiles (10: deferred stores; 11: previous stage first; 12: both, the
+//      FP32 default).
+//  variants 5, 6: the lean z-march with a cell-major LDS ring (FP32; kept for comparison).
 #include "physics.h"
 
 namespace ac {
@@ -604,21 +609,44 @@ solve_zmarch2(const SubstepArgs a, const int zdepth, const int ntx, const int nt
 // * Global loads and stores use a uniform per-plane base and a 32-bit per-thread byte offset.
 // * The right-hand side is formed field by field into the aggregates of MhdAgg, which keeps the
 //   live set small.
-template <int TX, int TY, bool COMP = false> struct ZMarchLean {
+// NSLOT = 4: the ring holds planes p-3..p and a step stages plane p into the slot just freed,
+// between two barriers. NSLOT = 5: plane p+1 is staged into a fifth slot while planes p-3..p are
+// read, so one barrier per step suffices (the staging registers then hold plane p+1 across the
+// step instead of plane p).
+template <int TX, int TY, bool COMP = false, int NSLOT = 4, bool DEFER = false,
+          bool PREV_FIRST = false>
+struct ZMarchLean {
     static constexpr int NT  = TX * TY;
     static constexpr int LX  = TX + 6, LY = TY + 6, LN = LX * LY;
     static constexpr int R   = (LN + NT - 1) / NT;
     static constexpr int LNP = R * NT;
 
     struct Shared {
-        AcReal ring[4][NF][LNP];
+        AcReal ring[NSLOT][NF][LNP];
     };
 
     struct State {
         AcReal qv[NF][3]; // planes z-3..z-1 of every field
         AcReal x[6][7];   // cross-derivative sums, slot (z - pbeg) mod 7
         AcReal pv[NF][R]; // staging registers of the next plane
+        AcReal res[NF];   // updated values of plane res_z, stored during the next step
+        int res_z;        // -1: nothing pending
     };
+
+    // Stores the output plane held in `res`. With DEFER it is issued at the start of the next
+    // step, before that step's loads: memory operations complete in issue order, so the wait for
+    // the next plane's loads does not also wait for the stores' acknowledgements.
+    __device__ __forceinline__ void flush(State& st) const
+    {
+        if (st.res_z >= 0 && active) {
+            const unsigned long oz = (unsigned long)st.res_z * a.mxy;
+            const unsigned ow      = pin(own);
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                *(AcReal*)((char*)(a.out[f] + oz) + ow) = st.res[f];
+        }
+        st.res_z = -1;
+    }
 
     const SubstepArgs& a;
     Shared& sh;
@@ -638,6 +666,18 @@ template <int TX, int TY, bool COMP = false> struct ZMarchLean {
     static __device__ __forceinline__ AcReal ld(const AcReal* base, unsigned byte_off)
     {
         return *(const AcReal*)((const char*)base + byte_off);
+    }
+
+    // Ring slot of plane q (q >= 0).
+    static __device__ __forceinline__ int ring_slot(int q) { return NSLOT == 4 ? (q & 3) : q % NSLOT; }
+
+    __device__ __forceinline__ void stage(const State& st, int slot) const
+    {
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                sh.ring[slot][f][r * NT + tid] = st.pv[f][r];
     }
 
     __device__ __forceinline__ void issue_loads(State& st, int p) const
@@ -680,17 +720,27 @@ template <int TX, int TY, bool COMP = false> struct ZMarchLean {
     template <int PH, bool OUT> __device__ __forceinline__ void step(State& st, int p) const
     {
         const PhysParams& P = a.phys;
-        const int slot      = p & 3;
+        const int slot      = ring_slot(p);
 
-        __syncthreads(); // every thread is done with ring slot `slot` (plane p - 4)
-#pragma unroll
-        for (int f = 0; f < NF; ++f)
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-                sh.ring[slot][f][r * NT + tid] = st.pv[f][r];
-        __syncthreads();
-        issue_loads(st, p + 1);
+        if (NSLOT == 4) {
+            __syncthreads(); // every thread is done with ring slot `slot` (plane p - 4)
+            stage(st, slot);
+            __syncthreads();
+        }
+        else {
+            // plane p (staged during step p - 1) is visible, and every thread is done with
+            // plane p - 4, whose slot now receives plane p + 1
+            __syncthreads();
+            stage(st, ring_slot(p + 1));
+        }
 
+        if (DEFER)
+            flush(st);
+        // With PREV_FIRST the previous stage's values are loaded before the next plane: memory
+        // operations complete in issue order, so waiting for them at the update then does not
+        // also wait for the next plane's loads (measured faster in FP64, slower in FP32).
+        if (!PREV_FIRST)
+            issue_loads(st, NSLOT == 4 ? p + 1 : p + 2);
         const int z             = p - 3;
         const unsigned long oz  = (unsigned long)min(z, ze - 1) * a.mxy;
         const unsigned ow       = pin(own);
@@ -700,6 +750,8 @@ template <int TX, int TY, bool COMP = false> struct ZMarchLean {
             for (int f = 0; f < NF; ++f)
                 prev[f] = ld(a.out[f] + oz, ow);
         }
+        if (PREV_FIRST)
+            issue_loads(st, NSLOT == 4 ? p + 1 : p + 2);
 
         // ---- ingest plane p: x-z and y-z differences into the pending sums ----
         const AcReal* __restrict__ rs = &sh.ring[slot][0][0];
@@ -742,7 +794,7 @@ template <int TX, int TY, bool COMP = false> struct ZMarchLean {
                 st.x[5][hi] -= d5;
             }
         }
-        const AcReal* __restrict__ ro = &sh.ring[z & 3][0][0];
+        const AcReal* __restrict__ ro = &sh.ring[ring_slot(p + NSLOT - 3)][0][0]; // plane z
         if (!OUT) {
             push_queue(st, ro);
             return;
@@ -750,8 +802,8 @@ template <int TX, int TY, bool COMP = false> struct ZMarchLean {
 
         // ---- output plane z = p - 3 ----
         constexpr int OS              = (PH + 4) % 7;
-        const AcReal* __restrict__ r1 = &sh.ring[(p + 2) & 3][0][0]; // plane z + 1
-        const AcReal* __restrict__ r2 = &sh.ring[(p + 3) & 3][0][0]; // plane z + 2
+        const AcReal* __restrict__ r1 = &sh.ring[ring_slot(p + NSLOT - 2)][0][0]; // plane z + 1
+        const AcReal* __restrict__ r2 = &sh.ring[ring_slot(p + NSLOT - 1)][0][0]; // plane z + 2
         MhdAgg g;
         AcReal v, dx, dy, dz, dxx, dyy, dzz;
 
@@ -819,21 +871,21 @@ template <int TX, int TY, bool COMP = false> struct ZMarchLean {
             store_compensated(st, rate, prev, oz, ow, z);
             return;
         }
-        if (z < ze && active) {
-            if (a.rk.step == 0) {
+        if (a.rk.step == 0) {
 #pragma unroll
-                for (int f = 0; f < NF; ++f)
-                    *(AcReal*)((char*)(a.out[f] + oz) + ow) = st.qv[f][2] + a.rk.beta_dt * rate[f];
-            }
-            else {
+            for (int f = 0; f < NF; ++f)
+                st.res[f] = st.qv[f][2] + a.rk.beta_dt * rate[f];
+        }
+        else {
 #pragma unroll
-                for (int f = 0; f < NF; ++f) {
-                    const AcReal cur = st.qv[f][2];
-                    *(AcReal*)((char*)(a.out[f] + oz) + ow) =
-                        cur + a.rk.beta * (a.rk.ab * (cur - prev[f]) + a.rk.dt * rate[f]);
-                }
+            for (int f = 0; f < NF; ++f) {
+                const AcReal cur = st.qv[f][2];
+                st.res[f] = cur + a.rk.beta * (a.rk.ab * (cur - prev[f]) + a.rk.dt * rate[f]);
             }
         }
+        st.res_z = z < ze ? z : -1;
+        if (!DEFER)
+            flush(st);
     }
 
     // Compensated update: the state is the pair (u, c) with u + c carrying the rounding error
@@ -1224,12 +1276,14 @@ solve_zaos(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
 #endif // AC_DOUBLE_PRECISION == 0
 
 // One z-march column: tile (btx, bty) of box `a`, output planes [zs, min(zs + zdepth, end.z)).
-template <int TX, int TY, bool COMP = false>
+template <int TX, int TY, bool COMP = false, int NSLOT = 4, bool DEFER = false,
+          bool PREV_FIRST = false>
 __device__ __forceinline__ void
-zlean_column(const SubstepArgs& a, typename ZMarchLean<TX, TY, COMP>::Shared& sh, const int btx,
-             const int bty, const int zs, const int zdepth)
+zlean_column(const SubstepArgs& a,
+             typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST>::Shared& sh,
+             const int btx, const int bty, const int zs, const int zdepth)
 {
-    typedef ZMarchLean<TX, TY, COMP> K;
+    typedef ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST> K;
     K k(a, sh);
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
@@ -1251,6 +1305,7 @@ zlean_column(const SubstepArgs& a, typename ZMarchLean<TX, TY, COMP>::Shared& sh
     k.cofs = (ty + 3) * K::LX + (tx + 3);
 
     typename K::State st;
+    st.res_z = -1;
     // Queue entries of planes below the march and sums of outputs below zs never reach a
     // stored output; zero them so no undefined value enters the arithmetic.
 #pragma unroll
@@ -1266,6 +1321,11 @@ zlean_column(const SubstepArgs& a, typename ZMarchLean<TX, TY, COMP>::Shared& sh
 
     const int pbeg = zs - 3;
     k.issue_loads(st, pbeg);
+    if (NSLOT == 5) {
+        __syncthreads(); // the previous column of this block is done with the ring
+        k.stage(st, K::ring_slot(pbeg));
+        k.issue_loads(st, pbeg + 1);
+    }
     k.template step<0, false>(st, pbeg);
     k.template step<1, false>(st, pbeg + 1);
     k.template step<2, false>(st, pbeg + 2);
@@ -1281,18 +1341,22 @@ zlean_column(const SubstepArgs& a, typename ZMarchLean<TX, TY, COMP>::Shared& sh
         k.template step<4, true>(st, p + 5);
         k.template step<5, true>(st, p + 6);
     }
+    if (DEFER)
+        k.flush(st); // the last output plane
 }
 
-template <int TX, int TY, int W, bool COMP = false>
+template <int TX, int TY, int W, bool COMP = false, int NSLOT = 4, bool DEFER = false,
+          bool PREV_FIRST = false>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W)))
 solve_zlean(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
 {
-    __shared__ typename ZMarchLean<TX, TY, COMP>::Shared sh;
+    __shared__ typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST>::Shared sh;
     const int b   = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int btx = b % ntx;
     const int bty = (b / ntx) % nty;
     const int btz = b / (ntx * nty);
-    zlean_column<TX, TY, COMP>(a, sh, btx, bty, a.start.z + btz * zdepth, zdepth);
+    zlean_column<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST>(a, sh, btx, bty, a.start.z + btz * zdepth,
+                                                         zdepth);
 }
 
 // Persistent form for ensembles of independent boxes: a grid sized to the resident block slots
@@ -1345,7 +1409,15 @@ constexpr ConfigEntry kConfigs[] = {
     {5, 16, 16},
     {6, 16, 16},
     {6, 32, 4},
+    {7, 32, 4},
+    {7, 16, 8},
+    {8, 32, 4},
+    {9, 32, 4},
+    {10, 16, 16},
+    {11, 16, 16},
+    {12, 16, 16},
 #endif
+    {8, 16, 16},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
@@ -1413,19 +1485,21 @@ kernel_config_at(int index)
 AcKernelConfig
 default_kernel_config(int nx, int ny, int nz)
 {
-    // The lean z-march (variant 3) measured fastest at 512^3: 32 x 4 tiles in FP32 (two blocks
-    // of 2 waves per CU), 16 x 16 in FP64 (one block of 4 waves per CU; its 128 KiB LDS ring
-    // leaves no room for a second). The z extent of a block is cut so that the grid still has
+    // Fastest at 512^3: the lean z-march that loads the previous stage first and defers its
+    // stores, on 16 x 16 tiles; in FP32 compiled for two waves per SIMD (variant 12: two blocks
+    // of 4 waves per CU, 64 KiB of LDS each), in FP64 for one (variant 8: its 128 KiB LDS ring
+    // leaves room for one block per CU). The z extent of a block is cut so that the grid still has
     // about four blocks per concurrently resident block slot, in multiples of the 7-plane round.
     AcKernelConfig c;
-    c.variant = 3;
 #if AC_DOUBLE_PRECISION == 0
-    c.tile_x = 32;
-    c.tile_y = 4;
-    const long want = 2048;
+    c.variant = 12;
+    c.tile_x  = 16;
+    c.tile_y  = 16;
+    const long want = 1024;
 #else
-    c.tile_x = 16;
-    c.tile_y = 16;
+    c.variant = 8;
+    c.tile_x  = 16;
+    c.tile_y  = 16;
     const long want = 1024;
 #endif
     const long cols = (long)((nx + c.tile_x - 1) / c.tile_x) * ((ny + c.tile_y - 1) / c.tile_y);
@@ -1497,7 +1571,30 @@ launch_substep(const AcKernelConfig& c, const SubstepArgs& a, hipStream_t stream
         return launch_zmarch_grid<16, 16>(solve_zaos<16, 16, 2>, a, c.z_depth, stream);
     if (c.variant == 6 && c.tile_x == 32 && c.tile_y == 4)
         return launch_zmarch_grid<32, 4>(solve_zaos<32, 4, 2>, a, c.z_depth, stream);
+    if (c.variant == 7 && c.tile_x == 32 && c.tile_y == 4)
+        return launch_zmarch_grid<32, 4>(solve_zlean<32, 4, 1, false, 5>, a, c.z_depth, stream);
+    if (c.variant == 7 && c.tile_x == 16 && c.tile_y == 8)
+        return launch_zmarch_grid<16, 8>(solve_zlean<16, 8, 1, false, 5>, a, c.z_depth, stream);
+    if (c.variant == 8 && c.tile_x == 32 && c.tile_y == 4)
+        return launch_zmarch_grid<32, 4>(solve_zlean<32, 4, 1, false, 4, true, true>, a, c.z_depth,
+                                         stream);
+    if (c.variant == 9 && c.tile_x == 32 && c.tile_y == 4)
+        return launch_zmarch_grid<32, 4>(solve_zlean<32, 4, 1, false, 5, true, true>, a, c.z_depth,
+                                         stream);
+    // two waves per SIMD (16 x 16 is the only tile whose 4-slot ring leaves room for two blocks)
+    if (c.variant == 10 && c.tile_x == 16 && c.tile_y == 16)
+        return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2, false, 4, true, false>, a,
+                                          c.z_depth, stream);
+    if (c.variant == 11 && c.tile_x == 16 && c.tile_y == 16)
+        return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2, false, 4, false, true>, a,
+                                          c.z_depth, stream);
+    if (c.variant == 12 && c.tile_x == 16 && c.tile_y == 16)
+        return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2, false, 4, true, true>, a,
+                                          c.z_depth, stream);
 #endif
+    if (c.variant == 8 && c.tile_x == 16 && c.tile_y == 16)
+        return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 1, false, 4, true, true>, a,
+                                          c.z_depth, stream);
 #undef AC_DIRECT
 #undef AC_ZMARCH
 #undef AC_ZMARCH2

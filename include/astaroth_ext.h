@@ -41,7 +41,13 @@ AcResult acDeviceIntegrateStepGraph(const Device device, const Stream stream, co
  *            the z stencils and accumulators for the x-z / y-z cross derivatives.
  * variant 2: variant 1 with the plane loop unrolled by the 7-plane queue length.
  * variants 3, 4: the lean z-march (all eight fields in one LDS ring, 3-register z queues, six
- *            combined cross-derivative sums), compiled for one and two waves per SIMD. */
+ *            combined cross-derivative sums), compiled for one and two waves per SIMD.
+ * variants 5, 6: the lean z-march with a cell-major LDS ring (FP32).
+ * variant 7: the lean z-march with a 5-slot ring, one barrier per plane.
+ * variants 8, 9: the lean z-march loading the previous stage first and storing each output
+ *            plane one step later; 4 and 5 ring slots (variant 8 is the FP64 default).
+ * variants 10-12: two-waves-per-SIMD builds of the lean z-march on 16 x 16 tiles (10: deferred
+ *            stores, 11: previous stage first, 12: both; variant 12 is the FP32 default). */
 typedef struct {
     int variant;
     int tile_x;
