@@ -613,9 +613,13 @@ solve_zmarch2(const SubstepArgs a, const int zdepth, const int ntx, const int nt
 // between two barriers. NSLOT = 5: plane p+1 is staged into a fifth slot while planes p-3..p are
 // read, so one barrier per step suffices (the staging registers then hold plane p+1 across the
 // step instead of plane p).
+// NQ: period of the cross-sum slots and of the unrolled plane loop. 7 is the minimum (outputs
+// p-3..p+3 are pending); with NQ = 8 and NSLOT = 4 the ring slot of every plane a step touches
+// is a compile-time function of the phase, so no LDS address is recomputed per step.
 template <int TX, int TY, bool COMP = false, int NSLOT = 4, bool DEFER = false,
-          bool PREV_FIRST = false>
+          bool PREV_FIRST = false, int NQ = 7>
 struct ZMarchLean {
+    static_assert(NQ == 7 || (NQ == 8 && NSLOT == 4), "8-phase loop needs the 4-slot ring");
     static constexpr int NT  = TX * TY;
     static constexpr int LX  = TX + 6, LY = TY + 6, LN = LX * LY;
     static constexpr int R   = (LN + NT - 1) / NT;
@@ -627,7 +631,7 @@ struct ZMarchLean {
 
     struct State {
         AcReal qv[NF][3]; // planes z-3..z-1 of every field
-        AcReal x[6][7];   // cross-derivative sums, slot (z - pbeg) mod 7
+        AcReal x[6][NQ];  // cross-derivative sums, slot (z - pbeg) mod NQ
         AcReal pv[NF][R]; // staging registers of the next plane
         AcReal res[NF];   // updated values of plane res_z, stored during the next step
         int res_z;        // -1: nothing pending
@@ -720,7 +724,8 @@ struct ZMarchLean {
     template <int PH, bool OUT> __device__ __forceinline__ void step(State& st, int p) const
     {
         const PhysParams& P = a.phys;
-        const int slot      = ring_slot(p);
+        // ring slot of plane p (and below, of planes z = p - 3, z + 1, z + 2)
+        const int slot = NQ == 8 ? (PH & 3) : ring_slot(p);
 
         if (NSLOT == 4) {
             __syncthreads(); // every thread is done with ring slot `slot` (plane p - 4)
@@ -757,7 +762,7 @@ struct ZMarchLean {
         const AcReal* __restrict__ rs = &sh.ring[slot][0][0];
 #pragma unroll
         for (int i = 1; i <= 3; ++i) {
-            const int lo = (PH - i + 7) % 7, hi = (PH + i) % 7; // outputs p - i and p + i
+            const int lo = (PH - i + NQ) % NQ, hi = (PH + i) % NQ; // outputs p - i and p + i
             const AcReal cx = P.cxz[i], cy = P.cyz[i];
             const AcReal* ux = rs + VTXBUF_UUX * LNP + cofs;
             const AcReal* uy = rs + VTXBUF_UUY * LNP + cofs;
@@ -794,16 +799,19 @@ struct ZMarchLean {
                 st.x[5][hi] -= d5;
             }
         }
-        const AcReal* __restrict__ ro = &sh.ring[ring_slot(p + NSLOT - 3)][0][0]; // plane z
+        const AcReal* __restrict__ ro =
+            &sh.ring[NQ == 8 ? ((PH + 1) & 3) : ring_slot(p + NSLOT - 3)][0][0]; // plane z
         if (!OUT) {
             push_queue(st, ro);
             return;
         }
 
         // ---- output plane z = p - 3 ----
-        constexpr int OS              = (PH + 4) % 7;
-        const AcReal* __restrict__ r1 = &sh.ring[ring_slot(p + NSLOT - 2)][0][0]; // plane z + 1
-        const AcReal* __restrict__ r2 = &sh.ring[ring_slot(p + NSLOT - 1)][0][0]; // plane z + 2
+        constexpr int OS = (PH + NQ - 3) % NQ; // slot of output z = p - 3
+        const AcReal* __restrict__ r1 =
+            &sh.ring[NQ == 8 ? ((PH + 2) & 3) : ring_slot(p + NSLOT - 2)][0][0]; // plane z + 1
+        const AcReal* __restrict__ r2 =
+            &sh.ring[NQ == 8 ? ((PH + 3) & 3) : ring_slot(p + NSLOT - 1)][0][0]; // plane z + 2
         MhdAgg g;
         AcReal v, dx, dy, dz, dxx, dyy, dzz;
 
@@ -1277,13 +1285,13 @@ solve_zaos(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
 
 // One z-march column: tile (btx, bty) of box `a`, output planes [zs, min(zs + zdepth, end.z)).
 template <int TX, int TY, bool COMP = false, int NSLOT = 4, bool DEFER = false,
-          bool PREV_FIRST = false>
+          bool PREV_FIRST = false, int NQ = 7>
 __device__ __forceinline__ void
 zlean_column(const SubstepArgs& a,
-             typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST>::Shared& sh,
+             typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ>::Shared& sh,
              const int btx, const int bty, const int zs, const int zdepth)
 {
-    typedef ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST> K;
+    typedef ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ> K;
     K k(a, sh);
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
@@ -1316,7 +1324,7 @@ zlean_column(const SubstepArgs& a,
 #pragma unroll
     for (int n = 0; n < 6; ++n)
 #pragma unroll
-        for (int j = 0; j < 7; ++j)
+        for (int j = 0; j < NQ; ++j)
             st.x[n][j] = 0;
 
     const int pbeg = zs - 3;
@@ -1332,31 +1340,45 @@ zlean_column(const SubstepArgs& a,
     k.template step<3, false>(st, pbeg + 3);
     k.template step<4, false>(st, pbeg + 4);
     k.template step<5, false>(st, pbeg + 5);
-    for (int p = pbeg + 6; p < k.ze + 3; p += 7) {
-        k.template step<6, true>(st, p);
-        k.template step<0, true>(st, p + 1);
-        k.template step<1, true>(st, p + 2);
-        k.template step<2, true>(st, p + 3);
-        k.template step<3, true>(st, p + 4);
-        k.template step<4, true>(st, p + 5);
-        k.template step<5, true>(st, p + 6);
+    if (NQ == 7) {
+        for (int p = pbeg + 6; p < k.ze + 3; p += 7) {
+            k.template step<6, true>(st, p);
+            k.template step<0, true>(st, p + 1);
+            k.template step<1, true>(st, p + 2);
+            k.template step<2, true>(st, p + 3);
+            k.template step<3, true>(st, p + 4);
+            k.template step<4, true>(st, p + 5);
+            k.template step<5, true>(st, p + 6);
+        }
+    }
+    else {
+        for (int p = pbeg + 6; p < k.ze + 3; p += 8) {
+            k.template step<6, true>(st, p);
+            k.template step<7 % NQ, true>(st, p + 1);
+            k.template step<0, true>(st, p + 2);
+            k.template step<1, true>(st, p + 3);
+            k.template step<2, true>(st, p + 4);
+            k.template step<3, true>(st, p + 5);
+            k.template step<4, true>(st, p + 6);
+            k.template step<5, true>(st, p + 7);
+        }
     }
     if (DEFER)
         k.flush(st); // the last output plane
 }
 
 template <int TX, int TY, int W, bool COMP = false, int NSLOT = 4, bool DEFER = false,
-          bool PREV_FIRST = false>
+          bool PREV_FIRST = false, int NQ = 7>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W)))
 solve_zlean(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
 {
-    __shared__ typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST>::Shared sh;
+    __shared__ typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ>::Shared sh;
     const int b   = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int btx = b % ntx;
     const int bty = (b / ntx) % nty;
     const int btz = b / (ntx * nty);
-    zlean_column<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST>(a, sh, btx, bty, a.start.z + btz * zdepth,
-                                                         zdepth);
+    zlean_column<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ>(a, sh, btx, bty,
+                                                             a.start.z + btz * zdepth, zdepth);
 }
 
 // Persistent form for ensembles of independent boxes: a grid sized to the resident block slots
@@ -1416,6 +1438,9 @@ constexpr ConfigEntry kConfigs[] = {
     {10, 16, 16},
     {11, 16, 16},
     {12, 16, 16},
+    {13, 16, 16},
+    {14, 32, 4},
+    {15, 16, 16},
 #endif
     {8, 16, 16},
 };
@@ -1590,6 +1615,16 @@ launch_substep(const AcKernelConfig& c, const SubstepArgs& a, hipStream_t stream
                                           c.z_depth, stream);
     if (c.variant == 12 && c.tile_x == 16 && c.tile_y == 16)
         return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2, false, 4, true, true>, a,
+                                          c.z_depth, stream);
+    // 8-phase plane loop (compile-time ring slots)
+    if (c.variant == 13 && c.tile_x == 16 && c.tile_y == 16)
+        return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2, false, 4, true, true, 8>, a,
+                                          c.z_depth, stream);
+    if (c.variant == 14 && c.tile_x == 32 && c.tile_y == 4)
+        return launch_zmarch_grid<32, 4>(solve_zlean<32, 4, 1, false, 4, false, false, 8>, a,
+                                         c.z_depth, stream);
+    if (c.variant == 15 && c.tile_x == 16 && c.tile_y == 16)
+        return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 1, false, 4, true, true, 8>, a,
                                           c.z_depth, stream);
 #endif
     if (c.variant == 8 && c.tile_x == 16 && c.tile_y == 16)
