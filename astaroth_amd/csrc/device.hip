@@ -1074,14 +1074,18 @@ acDeviceAutoOptimize(const Device device)
     const int iters  = cells >= (1L << 24) ? 3 : 5;
     AcKernelConfig best = default_kernel_config(nx_of(device), ny_of(device), nz_of(device));
     float best_ms       = INFINITY;
-    // z-depths: powers of two for variant 1, whole 7-plane rounds for the unrolled marches
+    // z-depths: powers of two for variant 1, whole rounds of the unrolled plane loop (7 planes,
+    // 8 for variants 13-15) for the other marches
     const int depths1[] = {8, 16, 32, 64, 128, 0};
     const int depths7[] = {14, 28, 56, 112, 0};
+    const int depths8[] = {16, 32, 64, 128, 0};
     for (int i = 0; i < kernel_config_count(); ++i) {
         AcKernelConfig c = kernel_config_at(i);
         if (c.tile_x > 2 * nx_of(device) || c.tile_y > 2 * ny_of(device))
             continue;
-        const int* depths = c.variant == 1 ? depths1 : depths7;
+        const int* depths = c.variant == 1                        ? depths1
+                            : (c.variant >= 13 && c.variant <= 15) ? depths8
+                                                                   : depths7;
         const int ndepth  = c.variant == 0 ? 1 : c.variant == 1 ? 6 : 5;
         for (int di = 0; di < ndepth; ++di) {
             c.z_depth = c.variant == 0 ? 0 : depths[di];

@@ -1511,28 +1511,31 @@ AcKernelConfig
 default_kernel_config(int nx, int ny, int nz)
 {
     // Fastest at 512^3: the lean z-march that loads the previous stage first and defers its
-    // stores, on 16 x 16 tiles; in FP32 compiled for two waves per SIMD (variant 12: two blocks
-    // of 4 waves per CU, 64 KiB of LDS each), in FP64 for one (variant 8: its 128 KiB LDS ring
-    // leaves room for one block per CU). The z extent of a block is cut so that the grid still has
-    // about four blocks per concurrently resident block slot, in multiples of the 7-plane round.
+    // stores, on 16 x 16 tiles; in FP32 compiled for two waves per SIMD with the 8-plane loop
+    // (variant 13: two blocks of 4 waves per CU, 64 KiB of LDS each), in FP64 for one wave per
+    // SIMD (variant 8: its 128 KiB LDS ring leaves room for one block per CU). The z extent of a
+    // block is cut so that the grid still has about four blocks per concurrently resident block
+    // slot, in multiples of the unrolled plane loop (8 or 7 planes).
     AcKernelConfig c;
 #if AC_DOUBLE_PRECISION == 0
-    c.variant = 12;
-    c.tile_x  = 16;
-    c.tile_y  = 16;
-    const long want = 1024;
+    c.variant        = 13;
+    c.tile_x         = 16;
+    c.tile_y         = 16;
+    const long want  = 1024;
+    const int period = 8;
 #else
-    c.variant = 8;
-    c.tile_x  = 16;
-    c.tile_y  = 16;
-    const long want = 1024;
+    c.variant        = 8;
+    c.tile_x         = 16;
+    c.tile_y         = 16;
+    const long want  = 1024;
+    const int period = 7;
 #endif
     const long cols = (long)((nx + c.tile_x - 1) / c.tile_x) * ((ny + c.tile_y - 1) / c.tile_y);
     const long nzb  = cols >= want ? 1 : (want + cols - 1) / cols;
     int zd          = (int)((nz + nzb - 1) / nzb);
-    zd              = ((zd + 6) / 7) * 7;
-    if (zd < 14)
-        zd = 14;
+    zd              = ((zd + period - 1) / period) * period;
+    if (zd < 2 * period)
+        zd = 2 * period;
     c.z_depth = zd >= nz ? 0 : zd;
     return c;
 }
