@@ -1074,8 +1074,11 @@ acDeviceAutoOptimize(const Device device)
     const int iters  = cells >= (1L << 24) ? 3 : 5;
     AcKernelConfig best = default_kernel_config(nx_of(device), ny_of(device), nz_of(device));
     float best_ms       = INFINITY;
+    // untimed launches first, so that the first configuration timed does not pay for the clocks
+    // ramping up from idle
+    (void)time_config(device, best, 4 * iters);
     // z-depths: powers of two for variant 1, whole rounds of the unrolled plane loop (7 planes,
-    // 8 for variants 13-15) for the other marches
+    // 8 for variant 13) for the other marches
     const int depths1[] = {8, 16, 32, 64, 128, 0};
     const int depths7[] = {14, 28, 56, 112, 0};
     const int depths8[] = {16, 32, 64, 128, 0};
@@ -1083,9 +1086,7 @@ acDeviceAutoOptimize(const Device device)
         AcKernelConfig c = kernel_config_at(i);
         if (c.tile_x > 2 * nx_of(device) || c.tile_y > 2 * ny_of(device))
             continue;
-        const int* depths = c.variant == 1                        ? depths1
-                            : (c.variant >= 13 && c.variant <= 15) ? depths8
-                                                                   : depths7;
+        const int* depths = c.variant == 1 ? depths1 : c.variant == 13 ? depths8 : depths7;
         const int ndepth  = c.variant == 0 ? 1 : c.variant == 1 ? 6 : 5;
         for (int di = 0; di < ndepth; ++di) {
             c.z_depth = c.variant == 0 ? 0 : depths[di];

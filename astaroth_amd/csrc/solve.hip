@@ -1283,6 +1283,21 @@ solve_zaos(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
 }
 #endif // AC_DOUBLE_PRECISION == 0
 
+// Tile row of thread `tid` in the lean z-march. LDS serves a 32-bit access 32 lanes at a time
+// over 32 banks. On a 16-wide tile the natural order puts rows y and y + 1 in one 32-lane group;
+// their LDS rows start 22 words (the padded row pitch) apart, so 6 of the 16 banks overlap and
+// every stencil read takes two passes. Here the two 16-lane halves of each group hold rows 8
+// apart instead: 8 * 22 = 176 = 16 mod 32, so each group covers all 32 banks once, for every
+// stencil offset. Row(tid) = 2 (tid / 64) + (tid / 32 mod 2) + 8 (tid / 16 mod 2).
+template <int TX, int TY>
+__device__ __forceinline__ int
+lean_tile_row(const int tid)
+{
+    if (TX == 16 && TY == 16)
+        return 2 * (tid >> 6) + ((tid >> 5) & 1) + 8 * ((tid >> 4) & 1);
+    return tid / TX;
+}
+
 // One z-march column: tile (btx, bty) of box `a`, output planes [zs, min(zs + zdepth, end.z)).
 template <int TX, int TY, bool COMP = false, int NSLOT = 4, bool DEFER = false,
           bool PREV_FIRST = false, int NQ = 7>
@@ -1294,7 +1309,7 @@ zlean_column(const SubstepArgs& a,
     typedef ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ> K;
     K k(a, sh);
     const int tid = threadIdx.x;
-    const int tx = tid % TX, ty = tid / TX;
+    const int tx = tid % TX, ty = lean_tile_row<TX, TY>(tid);
     const int x0 = a.start.x + btx * TX, y0 = a.start.y + bty * TY;
     k.tid        = tid;
     k.ze         = min(zs + zdepth, a.end.z);
@@ -1439,8 +1454,7 @@ constexpr ConfigEntry kConfigs[] = {
     {11, 16, 16},
     {12, 16, 16},
     {13, 16, 16},
-    {14, 32, 4},
-    {15, 16, 16},
+    {16, 16, 16},
 #endif
     {8, 16, 16},
 };
@@ -1511,18 +1525,18 @@ AcKernelConfig
 default_kernel_config(int nx, int ny, int nz)
 {
     // Fastest at 512^3: the lean z-march that loads the previous stage first and defers its
-    // stores, on 16 x 16 tiles; in FP32 compiled for two waves per SIMD with the 8-plane loop
-    // (variant 13: two blocks of 4 waves per CU, 64 KiB of LDS each), in FP64 for one wave per
-    // SIMD (variant 8: its 128 KiB LDS ring leaves room for one block per CU). The z extent of a
-    // block is cut so that the grid still has about four blocks per concurrently resident block
-    // slot, in multiples of the unrolled plane loop (8 or 7 planes).
+    // stores, on 16 x 16 tiles; in FP32 with a 5-slot ring (one barrier per plane) compiled for
+    // two waves per SIMD (variant 16: two blocks of 4 waves per CU, 80 KiB of LDS each), in FP64
+    // with a 4-slot ring for one wave per SIMD (variant 8: its 128 KiB ring leaves room for one
+    // block per CU). The z extent of a block is cut so that the grid still has about four blocks
+    // per concurrently resident block slot, in multiples of the unrolled 7-plane loop.
     AcKernelConfig c;
 #if AC_DOUBLE_PRECISION == 0
-    c.variant        = 13;
+    c.variant        = 16;
     c.tile_x         = 16;
     c.tile_y         = 16;
     const long want  = 1024;
-    const int period = 8;
+    const int period = 7;
 #else
     c.variant        = 8;
     c.tile_x         = 16;
@@ -1623,11 +1637,9 @@ launch_substep(const AcKernelConfig& c, const SubstepArgs& a, hipStream_t stream
     if (c.variant == 13 && c.tile_x == 16 && c.tile_y == 16)
         return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2, false, 4, true, true, 8>, a,
                                           c.z_depth, stream);
-    if (c.variant == 14 && c.tile_x == 32 && c.tile_y == 4)
-        return launch_zmarch_grid<32, 4>(solve_zlean<32, 4, 1, false, 4, false, false, 8>, a,
-                                         c.z_depth, stream);
-    if (c.variant == 15 && c.tile_x == 16 && c.tile_y == 16)
-        return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 1, false, 4, true, true, 8>, a,
+    // 5-slot ring (one barrier per plane), two waves per SIMD: 77 KiB of LDS per block
+    if (c.variant == 16 && c.tile_x == 16 && c.tile_y == 16)
+        return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2, false, 5, true, true>, a,
                                           c.z_depth, stream);
 #endif
     if (c.variant == 8 && c.tile_x == 16 && c.tile_y == 16)

@@ -48,9 +48,8 @@ AcResult acDeviceIntegrateStepGraph(const Device device, const Stream stream, co
  *            plane one step later; 4 and 5 ring slots (variant 8 is the FP64 default).
  * variants 10-12: two-waves-per-SIMD builds of the lean z-march on 16 x 16 tiles (10: deferred
  *            stores, 11: previous stage first, 12: both).
- * variants 13-15: the lean z-march with an 8-plane loop, whose ring slots are compile-time
- *            (FP32; 13: variant 12's options, two waves per SIMD, the FP32 default; 14: 32 x 4,
- *            one wave; 15: variant 12's options, one wave per SIMD). */
+ * variant 13: variant 12 with an 8-plane loop, whose LDS ring slots are compile-time (FP32).
+ * variant 16: variant 12 with the 5-slot ring (one barrier per plane; the FP32 default). */
 typedef struct {
     int variant;
     int tile_x;
