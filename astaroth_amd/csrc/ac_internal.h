@@ -94,6 +94,9 @@ bool kernel_config_supported(const AcKernelConfig& config);
 // Batched solve (persistent lean z-march over a work list): `boxes` and `items` are device
 // arrays; tile shapes 32x4, 16x8, 16x16.
 bool batch_tile_supported(int tile_x, int tile_y);
+// Blocks of the batched solve build for this tile that are resident at once on the current
+// device (occupancy per CU x CUs), or -1 on error.
+int batch_resident_blocks(int tile_x, int tile_y);
 hipError_t launch_substep_batch(int tile_x, int tile_y, const SubstepArgs* boxes,
                                 const BatchItem* items, int nitems, int nblocks,
                                 hipStream_t stream);

@@ -240,11 +240,11 @@ build_items(Ensemble e)
         AC_HIP_CHECK(hipMemcpy(e->d_items, w.data(), w.size() * sizeof(BatchItem),
                                hipMemcpyHostToDevice));
     }
-    // resident block slots of the persistent grid: one wave per SIMD
-    int cus = 0;
-    AC_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->id));
-    const int waves_per_block = (TX * TY + 63) / 64;
-    e->nblocks = std::max(1, cus * 4 / waves_per_block);
+    // the persistent grid: one block per resident block slot of the batched kernel
+    AC_HIP_CHECK(hipSetDevice(e->id));
+    e->nblocks = batch_resident_blocks(TX, TY);
+    if (e->nblocks < 1)
+        return AC_FAILURE;
     return AC_SUCCESS;
 }
 
