@@ -96,6 +96,22 @@ def structs(double: bool):
     return AcMeshInfo, AcMesh
 
 
+_SIMRES_CACHE = {}
+
+
+def simulation_result(double: bool):
+    """AcSimulationResult of include/astaroth_ext.h for one precision."""
+    if double not in _SIMRES_CACHE:
+        class AcSimulationResult(ctypes.Structure):
+            _fields_ = [("last_step", ctypes.c_int), ("t", real_ctype(double)), ("stop", ctypes.c_int)]
+
+        _SIMRES_CACHE[double] = AcSimulationResult
+    return _SIMRES_CACHE[double]
+
+
+STOP_REASONS = ("max_steps", "max_time", "small_dt", "nan", "stop_file")
+
+
 class KernelConfig(ctypes.Structure):
     """AcKernelConfig of include/astaroth_ext.h."""
 

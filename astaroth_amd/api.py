@@ -60,6 +60,7 @@ class Library:
         self.real = abi.real_ctype(double)
         self.dtype = abi.real_dtype(double)
         self.MeshInfo, self.Mesh = abi.structs(double)
+        self.SimulationResult = abi.simulation_result(double)
         self._declare()
         size_mi = ctypes.c_size_t()
         size_m = ctypes.c_size_t()
@@ -154,6 +155,15 @@ class Library:
             "acHostMeshRandomize": (i, [ctypes.POINTER(M)]),
             "acHostMeshDestroy": (i, [ctypes.POINTER(M)]),
             "acLoadConfig": (i, [ctypes.c_char_p, ctypes.POINTER(MI)]),
+            "acDeviceGetMeshInfo": (i, [vp, ctypes.POINTER(MI)]),
+            "acHostWriteMeshInfo": (i, [MI, ctypes.c_char_p]),
+            "acHostSaveMeshSnapshot": (i, [M, i, R, ctypes.c_char_p]),
+            "acHostReadMeshSnapshot": (i, [M, i, ctypes.POINTER(R), ctypes.c_char_p]),
+            "acTimeseriesBegin": (i, [ctypes.c_char_p, i]),
+            "acDeviceTimeseriesAppend": (i, [vp, i, i, R, R, ctypes.c_char_p, ctypes.POINTER(i)]),
+            "acDeviceSaveMeshSnapshot": (i, [vp, i, i, R, ctypes.c_char_p]),
+            "acDeviceLoadMeshSnapshot": (i, [vp, i, i, ctypes.POINTER(R), ctypes.c_char_p]),
+            "acDeviceRunSimulation": (i, [vp, ctypes.c_char_p, ctypes.POINTER(self.SimulationResult)]),
         }
         self.symbols = sorted(decl)
         for name, (res, args) in decl.items():
@@ -198,6 +208,25 @@ class Library:
         info.real_params[RP["AC_cs2_sound"]] = info.real_params[RP["AC_cs_sound"]] ** 2
         self.update_builtin_params(info)
         return info
+
+    # ---- snapshot files (host side; no GPU needed) ----------------------------------------
+    def write_mesh_info(self, info, dirpath: str):
+        if self.dll.acHostWriteMeshInfo(info, dirpath.encode()) != abi.AC_SUCCESS:
+            raise AstarothError("acHostWriteMeshInfo(%s) failed" % dirpath)
+
+    def save_mesh_snapshot(self, mesh: "HostMesh", step: int, t: float, dirpath: str):
+        if self.dll.acHostSaveMeshSnapshot(mesh.struct, int(step), self.real(t), dirpath.encode()) != abi.AC_SUCCESS:
+            raise AstarothError("acHostSaveMeshSnapshot(step %d, %s) failed" % (step, dirpath))
+
+    def read_mesh_snapshot(self, mesh: "HostMesh", step: int, dirpath: str) -> float:
+        t = self.real()
+        if self.dll.acHostReadMeshSnapshot(mesh.struct, int(step), ctypes.byref(t), dirpath.encode()) != abi.AC_SUCCESS:
+            raise AstarothError("acHostReadMeshSnapshot(step %d, %s) failed" % (step, dirpath))
+        return t.value
+
+    def timeseries_begin(self, path: str, start_step: int = 0):
+        if self.dll.acTimeseriesBegin(path.encode(), int(start_step)) != abi.AC_SUCCESS:
+            raise AstarothError("acTimeseriesBegin(%s) failed" % path)
 
     def device_create(self, device_id: int, info, autotune: bool | None = None) -> "Device":
         return Device(self, device_id, info, autotune)
@@ -380,6 +409,36 @@ class Device:
         out = ctypes.c_double()
         self._check(self.lib.dll.acDeviceTimestepCFL(self.handle, stream, ctypes.byref(out)), "acDeviceTimestepCFL")
         return out.value
+
+    def mesh_info(self):
+        info = self.lib.MeshInfo()
+        self._check(self.lib.dll.acDeviceGetMeshInfo(self.handle, ctypes.byref(info)), "acDeviceGetMeshInfo")
+        return info
+
+    def timeseries_append(self, step: int, dt: float, t: float, path: str, stream=abi.STREAM_DEFAULT) -> bool:
+        """Appends the diagnostics row of `step`; returns True when a statistic was NaN."""
+        nan = ctypes.c_int(0)
+        self._check(self.lib.dll.acDeviceTimeseriesAppend(self.handle, stream, int(step), self.lib.real(dt),
+                                                          self.lib.real(t), path.encode(), ctypes.byref(nan)),
+                    "acDeviceTimeseriesAppend")
+        return bool(nan.value)
+
+    def save_mesh_snapshot(self, step: int, t: float, dirpath: str, stream=abi.STREAM_DEFAULT):
+        self._check(self.lib.dll.acDeviceSaveMeshSnapshot(self.handle, stream, int(step), self.lib.real(t),
+                                                          dirpath.encode()), "acDeviceSaveMeshSnapshot")
+
+    def load_mesh_snapshot(self, step: int, dirpath: str, stream=abi.STREAM_DEFAULT) -> float:
+        t = self.lib.real()
+        self._check(self.lib.dll.acDeviceLoadMeshSnapshot(self.handle, stream, int(step), ctypes.byref(t),
+                                                          dirpath.encode()), "acDeviceLoadMeshSnapshot")
+        return t.value
+
+    def run_simulation(self, dirpath: str):
+        """acDeviceRunSimulation: returns (last step, t, stop reason)."""
+        r = self.lib.SimulationResult()
+        self._check(self.lib.dll.acDeviceRunSimulation(self.handle, dirpath.encode(), ctypes.byref(r)),
+                    "acDeviceRunSimulation")
+        return r.last_step, r.t, r.stop
 
     def set_compensated_integration(self, enable=True):
         return self._check(self.lib.dll.acDeviceSetCompensatedIntegration(self.handle, int(bool(enable))),

@@ -1195,4 +1195,13 @@ acDeviceGetStreamHandle(const Device device, const Stream stream)
     return (void*)device->streams[stream];
 }
 
+AcResult
+acDeviceGetMeshInfo(const Device device, AcMeshInfo* info)
+{
+    if (!device || !info)
+        return AC_FAILURE;
+    *info = device->uniforms;
+    return AC_SUCCESS;
+}
+
 } // extern "C"

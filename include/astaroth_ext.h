@@ -170,6 +170,62 @@ void* acEnsembleGetStreamHandle(const Ensemble ensemble);
 /* ---- Stream access for timing from the caller (returns the hipStream_t as void*) ---- */
 void* acDeviceGetStreamHandle(const Device device, const Stream stream);
 
+/* The parameters in effect on the Device (its creation configuration with every uniform loaded
+ * since). */
+AcResult acDeviceGetMeshInfo(const Device device, AcMeshInfo* info);
+
+/* ---- Snapshots, diagnostics time series and the time-stepping driver ----
+ * Files of the reference's standalone simulation (samples/standalone/simulation.cc), in `dir`
+ * (NULL or "" for the working directory):
+ *   mesh_info.list        every parameter by name (simulation.cc:62-110);
+ *   <FIELD>_<step>.mesh   the time stamp then the whole m^3 buffer, ghosts included, AcReal,
+ *                         x fastest, host byte order (simulation.cc:115-147, read 205-245);
+ *   timeseries.ts         title row and one row per diagnostics step (simulation.cc:251-313,
+ *                         345-362).
+ * The reference's reader (analysis/python/astar/data/read.py) and astaroth_amd/snapshot.py read
+ * them. I/O errors, missing files and files of the wrong length return AC_FAILURE. */
+AcResult acHostWriteMeshInfo(const AcMeshInfo info, const char* dir);
+AcResult acHostSaveMeshSnapshot(const AcMesh mesh, const int step, const AcReal t, const char* dir);
+AcResult acHostReadMeshSnapshot(AcMesh mesh, const int step, AcReal* t, const char* dir);
+/* Appends the title row when start_step == 0, then a line break (as the reference does on every
+ * start, also a resumed one). */
+AcResult acTimeseriesBegin(const char* path, const int start_step);
+/* Appends the row of `step`: step, t, dt, then min, rms, max of |u| and of every field over the
+ * interior (FP64 device reductions, rms = sqrt(sum / (nx ny nz)); synchronous). Sets *found_nan
+ * to 1 when a field's statistic is NaN. */
+AcResult acDeviceTimeseriesAppend(const Device device, const Stream stream, const int step,
+                                  const AcReal dt, const AcReal t, const char* path,
+                                  int* found_nan);
+/* Periodic fill of the `in` buffers, store to the host and write the snapshot of `step`. */
+AcResult acDeviceSaveMeshSnapshot(const Device device, const Stream stream, const int step,
+                                  const AcReal t, const char* dir);
+/* Read the snapshot of `step` into the `in` buffers (synchronous); *t gets its time stamp. */
+AcResult acDeviceLoadMeshSnapshot(const Device device, const Stream stream, const int step,
+                                  AcReal* t, const char* dir);
+
+typedef enum {
+    AC_STOP_MAX_STEPS = 0, /* reached AC_max_steps */
+    AC_STOP_MAX_TIME  = 1, /* t >= AC_max_time (> 0) */
+    AC_STOP_SMALL_DT  = 2, /* dt below 1e-5 of the early-run dt more than 10 steps in a row */
+    AC_STOP_NAN       = 3, /* a diagnostics row held NaN */
+    AC_STOP_FILE      = 4  /* a file named STOP exists in `dir` */
+} AcStopReason;
+
+typedef struct {
+    int last_step; /* last step taken */
+    AcReal t;      /* simulated time after it */
+    int stop;      /* AcStopReason */
+} AcSimulationResult;
+
+/* The reference driver loop (simulation.cc:316-560) on the Device's loaded state with periodic
+ * boundaries: with AC_start_step > 0 it first reads that step's snapshot; it writes
+ * mesh_info.list, appends to timeseries.ts (row 0 and snapshot 0 on a fresh start), then for
+ * steps start+1 .. AC_max_steps-1: dt = acDeviceTimestepCFL, acDeviceIntegrateStep, t += dt;
+ * a diagnostics row every AC_save_steps; a snapshot every AC_bin_steps or each time t passes the
+ * next multiple of AC_bin_save_t; it stops at AC_max_time, or with a final snapshot on a too
+ * small dt, a NaN or a STOP file. */
+AcResult acDeviceRunSimulation(const Device device, const char* dir, AcSimulationResult* result);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
