@@ -48,6 +48,7 @@ VTX = {n: i for i, n in enumerate(VTXBUF_NAMES)}
 RT = {n: i for i, n in enumerate(RTYPES)}
 
 AC_SUCCESS = 0
+AC_BOUNDCOND_PERIODIC, AC_BOUNDCOND_SYMMETRIC, AC_BOUNDCOND_ANTISYMMETRIC = 0, 1, 2
 AC_FAILURE = 1
 
 

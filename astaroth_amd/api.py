@@ -106,6 +106,7 @@ class Library:
             "acDevicePeriodicBoundcondStep": (i, [vp, i, i, I3, I3]),
             "acDevicePeriodicBoundconds": (i, [vp, i, I3, I3]),
             "acDeviceGeneralBoundconds": (i, [vp, i, I3, I3, MI, I3]),
+            "acDeviceGeneralBoundcondStep": (i, [vp, i, i, I3, I3, MI, I3]),
             "acDeviceReduceScal": (i, [vp, i, i, i, ctypes.POINTER(R)]),
             "acDeviceReduceVec": (i, [vp, i, i, i, i, i, ctypes.POINTER(R)]),
             "acDeviceReduceVecScal": (i, [vp, i, i, i, i, i, i, ctypes.POINTER(R)]),
@@ -156,6 +157,7 @@ class Library:
             "acHostMeshDestroy": (i, [ctypes.POINTER(M)]),
             "acLoadConfig": (i, [ctypes.c_char_p, ctypes.POINTER(MI)]),
             "acDeviceGetMeshInfo": (i, [vp, ctypes.POINTER(MI)]),
+            "acDeviceIntegrateStepGBC": (i, [vp, i, MI, R]),
             "acHostWriteMeshInfo": (i, [MI, ctypes.c_char_p]),
             "acHostSaveMeshSnapshot": (i, [M, i, R, ctypes.c_char_p]),
             "acHostReadMeshSnapshot": (i, [M, i, ctypes.POINTER(R), ctypes.c_char_p]),
@@ -360,6 +362,24 @@ class Device:
         start = start or (0, 0, 0)
         end = end or (nx + 6, ny + 6, nz + 6)
         return self.lib.dll.acDevicePeriodicBoundcondStep(self.handle, stream, handle, int3(*start), int3(*end))
+
+    def general_boundconds(self, config, start=None, end=None, bindex=(-1, -1, -1), handle=None,
+                           stream=abi.STREAM_DEFAULT):
+        """acDeviceGeneralBoundconds (all fields) or acDeviceGeneralBoundcondStep (one field) with
+        the face types of `config` (AC_bc_type_{bot,top}_{x,y,z})."""
+        nx, ny, nz = self.dims
+        start = int3(*(start or (0, 0, 0)))
+        end = int3(*(end or (nx + 6, ny + 6, nz + 6)))
+        if handle is None:
+            r = self.lib.dll.acDeviceGeneralBoundconds(self.handle, stream, start, end, config, int3(*bindex))
+        else:
+            r = self.lib.dll.acDeviceGeneralBoundcondStep(self.handle, stream, _vh(handle), start, end, config,
+                                                          int3(*bindex))
+        return self._check(r, "acDeviceGeneralBoundconds")
+
+    def integrate_step_gbc(self, config, dt, stream=abi.STREAM_DEFAULT):
+        return self._check(self.lib.dll.acDeviceIntegrateStepGBC(self.handle, stream, config, dt),
+                           "acDeviceIntegrateStepGBC")
 
     def integrate_step(self, dt, stream=abi.STREAM_DEFAULT, graph=False):
         fn = self.lib.dll.acDeviceIntegrateStepGraph if graph else self.lib.dll.acDeviceIntegrateStep

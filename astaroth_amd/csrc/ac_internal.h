@@ -107,6 +107,10 @@ AcKernelConfig default_kernel_config(int nx, int ny, int nz);
 // Periodic fill of the ghost cells inside [start, end) of `nfields` buffers (boundconds.hip).
 hipError_t launch_periodic_fill(AcReal* const* fields, int nfields, int nx, int ny, int nz,
                                 int3 start, int3 end, hipStream_t stream);
+// Ghost cells of [start, end) by face type (AC_BOUNDCOND_*; bot/top per axis x, y, z).
+hipError_t launch_general_fill(AcReal* const* fields, int nfields, int nx, int ny, int nz,
+                               int3 start, int3 end, const int bot[3], const int top[3],
+                               hipStream_t stream);
 
 // Reductions (reductions.hip). `vals` are 1, 3 or 4 field pointers. Writes the FP64 result to
 // *d_result (device) and returns the launch status.

@@ -860,27 +860,68 @@ acDeviceHaloExchangeSelf(const Device device, const Stream stream)
     return acDeviceUnpackHalo(device, stream, device->halo_buffer);
 }
 
+// ref device.cc:457-475 over boundconds.cuh:192-242; the face types come from `config`
+// (AC_bc_type_{bot,top}_{x,y,z}). bindex = (-1, -1, -1) is the single-device form the Node layer
+// passes (node.cc:672-690): this Device owns every face. The reference's multi-process forms
+// (bindex >= 0, disabled there) are not provided and return AC_FAILURE, as do unknown face
+// types and mirrored axes shorter than 4 cells (their mirror images would be ghost cells).
+static bool
+general_bc_faces(const Device device, const AcMeshInfo& config, const int3 bindex, int bot[3],
+                 int top[3])
+{
+    if (bindex.x >= 0 || bindex.y >= 0 || bindex.z >= 0) {
+        fprintf(stderr, "acDeviceGeneralBoundcond*: only bindex = (-1, -1, -1) is provided\n");
+        return false;
+    }
+    const AcIntParam b[3] = {AC_bc_type_bot_x, AC_bc_type_bot_y, AC_bc_type_bot_z};
+    const AcIntParam t[3] = {AC_bc_type_top_x, AC_bc_type_top_y, AC_bc_type_top_z};
+    const int n[3]        = {nx_of(device), ny_of(device), nz_of(device)};
+    for (int d = 0; d < 3; ++d) {
+        bot[d] = config.int_params[b[d]];
+        top[d] = config.int_params[t[d]];
+        for (int v : {bot[d], top[d]}) {
+            if (v != AC_BOUNDCOND_PERIODIC && v != AC_BOUNDCOND_SYMMETRIC &&
+                v != AC_BOUNDCOND_ANTISYMMETRIC) {
+                fprintf(stderr, "acDeviceGeneralBoundcond*: unknown boundary type %d\n", v);
+                return false;
+            }
+            if (v != AC_BOUNDCOND_PERIODIC && n[d] < 4)
+                return false;
+        }
+        if (n[d] < NGHOST)
+            return false;
+    }
+    return true;
+}
+
 AcResult
 acDeviceGeneralBoundcondStep(const Device device, const Stream stream,
                              const VertexBufferHandle vtxbuf_handle, const int3 start,
                              const int3 end, const AcMeshInfo config, const int3 bindex)
 {
-    (void)device;
-    (void)stream;
-    (void)vtxbuf_handle;
-    (void)start;
-    (void)end;
-    (void)config;
-    (void)bindex;
-    fprintf(stderr, "acDeviceGeneralBoundcondStep: only periodic boundaries are provided\n");
-    return AC_FAILURE;
+    int bot[3], top[3];
+    if (!stream_ok(device, stream) || vtxbuf_handle < 0 || vtxbuf_handle >= NF ||
+        !general_bc_faces(device, config, bindex, bot, top))
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(device->id));
+    AcReal* f[1] = {device->vba.in[vtxbuf_handle]};
+    AC_HIP_CHECK(launch_general_fill(f, 1, nx_of(device), ny_of(device), nz_of(device), start,
+                                     end, bot, top, device->streams[stream]));
+    return AC_SUCCESS;
 }
 
 AcResult
 acDeviceGeneralBoundconds(const Device device, const Stream stream, const int3 start,
                           const int3 end, const AcMeshInfo config, const int3 bindex)
 {
-    return acDeviceGeneralBoundcondStep(device, stream, VTXBUF_LNRHO, start, end, config, bindex);
+    int bot[3], top[3];
+    if (!stream_ok(device, stream) || !general_bc_faces(device, config, bindex, bot, top))
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(device->id));
+    AC_HIP_CHECK(launch_general_fill(device->vba.in, NF, nx_of(device), ny_of(device),
+                                     nz_of(device), start, end, bot, top,
+                                     device->streams[stream]));
+    return AC_SUCCESS;
 }
 
 AcResult
@@ -962,6 +1003,26 @@ acDeviceIntegrateStep(const Device device, const Stream stream, const AcReal dt)
     const int3 nmax = mk3(nx_of(device) + NGHOST, ny_of(device) + NGHOST, nz_of(device) + NGHOST);
     for (int s = 0; s < 3; ++s) {
         if (acDevicePeriodicBoundconds(device, stream, zero, m) != AC_SUCCESS ||
+            acDeviceIntegrateSubstep(device, stream, s, nmin, nmax, dt) != AC_SUCCESS)
+            return AC_FAILURE;
+        acDeviceSwapBuffers(device);
+    }
+    return AC_SUCCESS;
+}
+
+// The step of acNodeIntegrateGBC (node.cc:808-880) on one Device: 3 x {general boundary fill by
+// the face types of `config`, substep over the interior, swap}.
+AcResult
+acDeviceIntegrateStepGBC(const Device device, const Stream stream, const AcMeshInfo config,
+                         const AcReal dt)
+{
+    const int3 zero  = mk3(0, 0, 0);
+    const int3 m     = mk3(nx_of(device) + 6, ny_of(device) + 6, nz_of(device) + 6);
+    const int3 nmin  = mk3(NGHOST, NGHOST, NGHOST);
+    const int3 nmax  = mk3(nx_of(device) + NGHOST, ny_of(device) + NGHOST, nz_of(device) + NGHOST);
+    const int3 owner = mk3(-1, -1, -1);
+    for (int s = 0; s < 3; ++s) {
+        if (acDeviceGeneralBoundconds(device, stream, zero, m, config, owner) != AC_SUCCESS ||
             acDeviceIntegrateSubstep(device, stream, s, nmin, nmax, dt) != AC_SUCCESS)
             return AC_FAILURE;
         acDeviceSwapBuffers(device);

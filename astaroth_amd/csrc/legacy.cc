@@ -8,9 +8,12 @@
 //   acReduce*      = the Device reduction finalised as the Node does (node.cc:932-957):
 //                    RMS, RMS_EXP and ALFVEN_RMS return sqrt(sum / n_x n_y n_z),
 //   acSynchronizeMesh has nothing to exchange.
-// General (non-periodic) boundaries are outside this library: the *GBC calls return AC_FAILURE.
+//   acIntegrateGBC = 3 x {general fill by the face types of `config`, substep, swap}
+//                    (node.cc:808-880), acBoundcondStepGBC = the general fill alone
+//                    (node.cc:923-930), both with the corrected fill of boundconds.hip.
 // The Node handle itself (acGetNode) is not provided.
 #include "astaroth.h"
+#include "astaroth_ext.h"
 
 #include <cmath>
 #include <cstdio>
@@ -195,10 +198,12 @@ acIntegrate(const AcReal dt)
 AcResult
 acIntegrateGBC(const AcMeshInfo config, const AcReal dt)
 {
-    (void)config;
-    (void)dt;
-    fprintf(stderr, "acIntegrateGBC: only periodic boundaries are provided\n");
-    return AC_FAILURE;
+    if (!ready("acIntegrateGBC"))
+        return AC_FAILURE;
+    acDeviceSynchronizeStream(g_device, STREAM_ALL);
+    const AcResult r = acDeviceIntegrateStepGBC(g_device, STREAM_DEFAULT, config, dt);
+    acDeviceSynchronizeStream(g_device, STREAM_ALL);
+    return r;
 }
 
 AcResult
@@ -235,9 +240,12 @@ acBoundcondStep(void)
 AcResult
 acBoundcondStepGBC(const AcMeshInfo config)
 {
-    (void)config;
-    fprintf(stderr, "acBoundcondStepGBC: only periodic boundaries are provided\n");
-    return AC_FAILURE;
+    if (!ready("acBoundcondStepGBC"))
+        return AC_FAILURE;
+    const int3 m = i3(g_info.int_params[AC_nx] + 2 * NGHOST, g_info.int_params[AC_ny] + 2 * NGHOST,
+                      g_info.int_params[AC_nz] + 2 * NGHOST);
+    return acDeviceGeneralBoundconds(g_device, STREAM_DEFAULT, i3(0, 0, 0), m, config,
+                                     i3(-1, -1, -1));
 }
 
 AcReal

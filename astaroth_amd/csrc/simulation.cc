@@ -14,9 +14,10 @@
 //
 // The reference's analysis package reads these files (analysis/python/astar/data/read.py:33-66,
 // 612-652); astaroth_amd/snapshot.py reads them the same way. Differences from the reference:
-// boundaries are periodic (the reference driver's general boundary conditions are not part of
-// this library), the diagnostics are FP64-accumulated device reductions, file errors return
-// AC_FAILURE instead of printing and continuing, and a snapshot read checks the file length.
+// the boundary fill is the corrected general fill (acDeviceGeneralBoundconds; periodic unless
+// the configuration says otherwise), the diagnostics are FP64-accumulated device reductions,
+// file errors return AC_FAILURE instead of printing and continuing, and a snapshot read checks
+// the file length.
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -211,7 +212,7 @@ acDeviceSaveMeshSnapshot(const Device device, const Stream stream, const int ste
     AcMesh host;
     if (acHostMeshCreate(info, &host) != AC_SUCCESS)
         return AC_FAILURE;
-    AcResult r = acDevicePeriodicBoundconds(device, stream, mk3(0, 0, 0), m);
+    AcResult r = acDeviceGeneralBoundconds(device, stream, mk3(0, 0, 0), m, info, mk3(-1, -1, -1));
     if (r == AC_SUCCESS)
         r = acDeviceStoreMesh(device, stream, &host);
     if (r == AC_SUCCESS)
@@ -280,7 +281,7 @@ acDeviceRunSimulation(const Device device, const char* dir, AcSimulationResult* 
         if (acDeviceTimestepCFL(device, s, &dtd) != AC_SUCCESS)
             return AC_FAILURE;
         const AcReal dt = (AcReal)dtd;
-        if (acDeviceIntegrateStep(device, s, dt) != AC_SUCCESS)
+        if (acDeviceIntegrateStepGBC(device, s, info, dt) != AC_SUCCESS)
             return AC_FAILURE;
         t += dt;
         last = i;

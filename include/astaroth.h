@@ -251,8 +251,12 @@ AcResult acDevicePeriodicBoundcondStep(const Device device, const Stream stream,
 AcResult acDevicePeriodicBoundconds(const Device device, const Stream stream, const int3 start,
                                     const int3 end);
 
-/* ref :621-628. Symmetric/antisymmetric boundaries are not provided by this library: these
- * return AC_FAILURE. */
+/* ref :621-628. Ghost cells of `in` within [start, end) by the face types of `config`
+ * (AC_bc_type_{bot,top}_{x,y,z}: AC_BOUNDCOND_PERIODIC wraps, _SYMMETRIC mirrors about the
+ * first/last interior cell, _ANTISYMMETRIC mirrors with a sign change; edges and corners combine
+ * their axes). bindex must be (-1, -1, -1), the single-device form; mirrored axes need n >= 4.
+ * The reference's kernel (boundconds.cuh:3-128) writes 1.0 into every ghost cell and reads only
+ * the x-top type; this is the fill its comments describe. */
 AcResult acDeviceGeneralBoundcondStep(const Device device, const Stream stream,
                                       const VertexBufferHandle vtxbuf_handle, const int3 start,
                                       const int3 end, const AcMeshInfo config, const int3 bindex);
@@ -288,8 +292,8 @@ AcResult acHostMeshDestroy(AcMesh* mesh);
 /* ---- Legacy interface (ref :208-283, src/core/astaroth.cc:28-188) over one Device ----
  * acIntegrate is 3 x {periodic fill, substep, swap} and synchronises; acReduce* finalise the
  * RMS types as sqrt(sum / n_x n_y n_z), as the reference's Node does (node.cc:932-957);
- * acLoad is asynchronous on STREAM_DEFAULT, acStore synchronous. The *GBC variants (general
- * boundaries) return AC_FAILURE. */
+ * acLoad is asynchronous on STREAM_DEFAULT, acStore synchronous. The *GBC variants use the
+ * general boundary fill of acDeviceGeneralBoundconds with the face types of their `config`. */
 AcResult acInit(const AcMeshInfo mesh_info);
 AcResult acQuit(void);
 AcResult acCheckDeviceAvailability(void);

@@ -32,6 +32,12 @@ AcResult acDeviceReduceVecScalDouble(const Device device, const Stream stream,
  * The state ends in `in` without a final fill, as in the reference. */
 AcResult acDeviceIntegrateStep(const Device device, const Stream stream, const AcReal dt);
 
+/* The step with general boundaries (acNodeIntegrateGBC, node.cc:808-880, on one Device):
+ * 3 x {acDeviceGeneralBoundconds over the whole box with the face types of `config`, substep,
+ * swap}. */
+AcResult acDeviceIntegrateStepGBC(const Device device, const Stream stream,
+                                  const AcMeshInfo config, const AcReal dt);
+
 /* Same step recorded once into a HIP graph per (stream, dt) and replayed afterwards. */
 AcResult acDeviceIntegrateStepGraph(const Device device, const Stream stream, const AcReal dt);
 
@@ -196,7 +202,8 @@ AcResult acTimeseriesBegin(const char* path, const int start_step);
 AcResult acDeviceTimeseriesAppend(const Device device, const Stream stream, const int step,
                                   const AcReal dt, const AcReal t, const char* path,
                                   int* found_nan);
-/* Periodic fill of the `in` buffers, store to the host and write the snapshot of `step`. */
+/* Boundary fill of the `in` buffers by the Device's face types (acDeviceGeneralBoundconds),
+ * store to the host and write the snapshot of `step`. */
 AcResult acDeviceSaveMeshSnapshot(const Device device, const Stream stream, const int step,
                                   const AcReal t, const char* dir);
 /* Read the snapshot of `step` into the `in` buffers (synchronous); *t gets its time stamp. */
@@ -217,8 +224,9 @@ typedef struct {
     int stop;      /* AcStopReason */
 } AcSimulationResult;
 
-/* The reference driver loop (simulation.cc:316-560) on the Device's loaded state with periodic
- * boundaries: with AC_start_step > 0 it first reads that step's snapshot; it writes
+/* The reference driver loop (simulation.cc:316-560) on the Device's loaded state, with the
+ * Device's boundary face types (acDeviceIntegrateStepGBC; periodic by default): with
+ * AC_start_step > 0 it first reads that step's snapshot; it writes
  * mesh_info.list, appends to timeseries.ts (row 0 and snapshot 0 on a fresh start), then for
  * steps start+1 .. AC_max_steps-1: dt = acDeviceTimestepCFL, acDeviceIntegrateStep, t += dt;
  * a diagnostics row every AC_save_steps; a snapshot every AC_bin_steps or each time t passes the
