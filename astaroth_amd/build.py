@@ -25,7 +25,7 @@ INCLUDE = os.path.join(REPO, "include")
 
 HIP_SOURCES = ["device.hip", "solve.hip", "boundconds.hip", "reductions.hip", "init.hip", "halo.hip",
                "ensemble.hip"]
-HOST_SOURCES = ["host.cc", "legacy.cc", "simulation.cc"]
+HOST_SOURCES = ["host.cc", "legacy.cc", "simulation.cc", "fortran.cc"]
 ARCH = "gfx950"
 
 
@@ -121,6 +121,10 @@ def main(argv=None):
         precs.append(False)
     if a.fp64 or not a.fp32:
         precs.append(True)
+    from . import fortran_include
+
+    for p in fortran_include.write():
+        print("wrote", p)
     for d in precs:
         print("built", build_precision(d, a.j, a.v))
     if not a.no_oracle:

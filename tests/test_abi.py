@@ -20,7 +20,7 @@ def declared_functions(header):
     text = open(os.path.join(REPO, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     text = re.sub(r"static inline[^{]*\{.*?\n\}", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:AcResult|AcReal|int|float|void\s*\*|Error|bool)\s+(ac\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:AcResult|AcReal|int|float|void\s*\*|void|Error|bool)\s+(ac\w+)\s*\(", text, re.M)))
 
 
 @pytest.fixture(params=[False, True], ids=["fp32", "fp64"])
@@ -31,8 +31,10 @@ def lib(request):
 
 
 def test_exports_every_declared_symbol(lib):
-    for header in ("astaroth.h", "astaroth_ext.h"):
-        for name in declared_functions(header):
+    for header in ("astaroth.h", "astaroth_ext.h", "astaroth_fortran.h"):
+        names = declared_functions(header)
+        assert names, header
+        for name in names:
             assert hasattr(lib.dll, name), (header, name)
     # acLoadConfig is the one utils function the library provides
     assert hasattr(lib.dll, "acLoadConfig")
