@@ -43,47 +43,48 @@ identity(int op)
     return op == OP_MAX ? -INFINITY : op == OP_MIN ? INFINITY : 0.0;
 }
 
+// Filter of one cell given its NV values (NV = 1, 3 or 4 fields), for reduction type RT.
+template <int NV, int RT>
 __device__ __forceinline__ double
-filter(const RedArgs& a, long idx)
+filter(const AcReal* x)
 {
     // No FMA contraction: MAX/MIN of vector lengths must round like the host's a*a + b*b + c*c.
 #pragma clang fp contract(off)
     const double kFourPi = 12.566370614359172953850573533118;
-    if (a.nvals == 1) {
-        const AcReal x = a.v[0][idx];
-        switch (a.rtype) {
+    if (NV == 1) {
+        switch (RT) {
         case RTYPE_MAX:
         case RTYPE_MIN:
-        case RTYPE_SUM: return (double)x;
-        case RTYPE_RMS: return (double)x * (double)x;
+        case RTYPE_SUM: return (double)x[0];
+        case RTYPE_RMS: return (double)x[0] * (double)x[0];
         case RTYPE_RMS_EXP: {
-            const double e = exp((double)x);
+            const double e = exp((double)x[0]);
             return e * e;
         }
         }
     }
-    else if (a.nvals == 3) {
-        const AcReal x = a.v[0][idx], y = a.v[1][idx], z = a.v[2][idx];
-        switch (a.rtype) {
+    else if (NV == 3) {
+        switch (RT) {
         case RTYPE_MAX:
-        case RTYPE_MIN: return (double)sqrt(x * x + y * y + z * z);
-        case RTYPE_SUM: return sqrt((double)x * x + (double)y * y + (double)z * z);
-        case RTYPE_RMS: return (double)x * x + (double)y * y + (double)z * z;
+        case RTYPE_MIN: return (double)sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+        case RTYPE_SUM:
+            return sqrt((double)x[0] * x[0] + (double)x[1] * x[1] + (double)x[2] * x[2]);
+        case RTYPE_RMS: return (double)x[0] * x[0] + (double)x[1] * x[1] + (double)x[2] * x[2];
         case RTYPE_RMS_EXP: {
-            const double ex = exp((double)x), ey = exp((double)y), ez = exp((double)z);
+            const double ex = exp((double)x[0]), ey = exp((double)x[1]), ez = exp((double)x[2]);
             return ex * ex + ey * ey + ez * ez;
         }
         }
     }
     else {
-        const AcReal x = a.v[0][idx], y = a.v[1][idx], z = a.v[2][idx], d = a.v[3][idx];
-        switch (a.rtype) {
+        switch (RT) {
         case RTYPE_ALFVEN_MAX:
         case RTYPE_ALFVEN_MIN:
-            return sqrt((double)x * x + (double)y * y + (double)z * z) /
-                   sqrt(kFourPi * exp((double)d));
+            return sqrt((double)x[0] * x[0] + (double)x[1] * x[1] + (double)x[2] * x[2]) /
+                   sqrt(kFourPi * exp((double)x[3]));
         case RTYPE_ALFVEN_RMS:
-            return ((double)x * x + (double)y * y + (double)z * z) / (kFourPi * exp((double)d));
+            return ((double)x[0] * x[0] + (double)x[1] * x[1] + (double)x[2] * x[2]) /
+                   (kFourPi * exp((double)x[3]));
         }
     }
     return NAN;
@@ -114,24 +115,42 @@ block_reduce(int op, double v, double* lds)
     return r;
 }
 
+// Each wave walks whole x rows (grid-stride over (j, k)). A lane loads U cells of a row (U x NV
+// values) before it folds any of them, so that every wave keeps U x NV loads in flight: with
+// one load per fold the walk ran at 2.2 TB/s, bound by memory latency rather than bandwidth;
+// now one 512^3 field takes 140 us in FP32 (3.8 TB/s) and 181 us in FP64 (5.9 TB/s), three
+// fields 266 us (6.1 TB/s) and 553 us (5.8 TB/s).
+template <int NV, int RT, int OP, int U>
 __global__ void __launch_bounds__(RED_THREADS)
-reduce_rows(const RedArgs a, int op, double* partials)
+reduce_rows(const RedArgs a, double* partials)
 {
     __shared__ double lds[RED_WAVES];
     const int lane  = threadIdx.x & 63;
     const int gwave = blockIdx.x * RED_WAVES + (threadIdx.x >> 6);
     const int nwave = gridDim.x * RED_WAVES;
     const int nrows = a.ny * a.nz;
-    double acc = identity(op);
+    double acc      = identity(OP);
     for (int r = gwave; r < nrows; r += nwave) {
         const int j = 3 + r % a.ny, k = 3 + r / a.ny;
         const long row = (long)j * a.mx + (long)k * a.mxy + 3;
-        for (int i = lane; i < a.nx; i += 64)
-            acc = combine(op, acc, filter(a, row + i));
+        for (int i0 = lane; i0 < a.nx; i0 += 64 * U) {
+            AcReal x[U][NV];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + 64 * u;
+#pragma unroll
+                for (int q = 0; q < NV; ++q)
+                    x[u][q] = i < a.nx ? a.v[q][row + i] : AcReal(0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (i0 + 64 * u < a.nx)
+                    acc = combine(OP, acc, filter<NV, RT>(x[u]));
+        }
     }
-    const double r = block_reduce(op, acc, lds);
+    const double res = block_reduce(OP, acc, lds);
     if (threadIdx.x == 0)
-        partials[blockIdx.x] = r;
+        partials[blockIdx.x] = res;
 }
 
 __global__ void __launch_bounds__(RED_THREADS)
@@ -183,7 +202,28 @@ launch_reduce(int rtype, int nvals, const AcReal* const* vals, const AcMeshInfo&
         blocks = maxblk;
     if (blocks < 1)
         blocks = 1;
-    reduce_rows<<<dim3(blocks), dim3(RED_THREADS), 0, stream>>>(a, op, d_partials);
+    const dim3 g(blocks), t(RED_THREADS);
+#define AC_RED(NV, RT, OP, U)                                                                      \
+    if (nvals == NV && rtype == RT) {                                                              \
+        reduce_rows<NV, RT, OP, U><<<g, t, 0, stream>>>(a, d_partials);                            \
+    } else
+    // cells per lane and pass (measured at 512^3: 8 / 4; 16 / 8 were slower in FP32)
+    constexpr int U1 = 8, U3 = 4;
+    AC_RED(1, RTYPE_MAX, OP_MAX, U1)
+    AC_RED(1, RTYPE_MIN, OP_MIN, U1)
+    AC_RED(1, RTYPE_SUM, OP_SUM, U1)
+    AC_RED(1, RTYPE_RMS, OP_SUM, U1)
+    AC_RED(1, RTYPE_RMS_EXP, OP_SUM, U1)
+    AC_RED(3, RTYPE_MAX, OP_MAX, U3)
+    AC_RED(3, RTYPE_MIN, OP_MIN, U3)
+    AC_RED(3, RTYPE_SUM, OP_SUM, U3)
+    AC_RED(3, RTYPE_RMS, OP_SUM, U3)
+    AC_RED(3, RTYPE_RMS_EXP, OP_SUM, U3)
+    AC_RED(4, RTYPE_ALFVEN_MAX, OP_MAX, U3)
+    AC_RED(4, RTYPE_ALFVEN_MIN, OP_MIN, U3)
+    AC_RED(4, RTYPE_ALFVEN_RMS, OP_SUM, U3)
+    return hipErrorInvalidValue;
+#undef AC_RED
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
