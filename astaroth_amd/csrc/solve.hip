@@ -132,10 +132,10 @@ constexpr int R_UX = 0, R_UY = 1, R_UZ = 2, R_AX = 3, R_AY = 4, R_AZ = 5;
 constexpr int XZ_FIELDS[4] = {R_UX, R_UZ, R_AX, R_AZ};
 constexpr int YZ_FIELDS[4] = {R_UY, R_UZ, R_AY, R_AZ};
 
-template <int LX>
+template <int LX, typename Ptr>
 __device__ __forceinline__ void
-inplane_lds(const AcReal* __restrict__ s, int c, const PhysParams& p, AcReal v, AcReal& dx,
-            AcReal& dy, AcReal& dxx, AcReal& dyy)
+inplane_lds(Ptr s, int c, const PhysParams& p, AcReal v, AcReal& dx, AcReal& dy, AcReal& dxx,
+            AcReal& dyy)
 {
     const AcReal xp1 = s[c + 1], xm1 = s[c - 1], xp2 = s[c + 2], xm2 = s[c - 2];
     const AcReal xp3 = s[c + 3], xm3 = s[c - 3];
@@ -149,9 +149,9 @@ inplane_lds(const AcReal* __restrict__ s, int c, const PhysParams& p, AcReal v, 
           p.d2[1][3] * (yp3 + ym3);
 }
 
-template <int LX>
+template <int LX, typename Ptr>
 __device__ __forceinline__ AcReal
-cross_xy_lds(const AcReal* __restrict__ s, int c, const AcReal* cxy)
+cross_xy_lds(Ptr s, int c, const AcReal* cxy)
 {
     AcReal res = 0;
 #pragma unroll
@@ -616,8 +616,11 @@ solve_zmarch2(const SubstepArgs a, const int zdepth, const int ntx, const int nt
 // NQ: period of the cross-sum slots and of the unrolled plane loop. 7 is the minimum (outputs
 // p-3..p+3 are pending); with NQ = 8 and NSLOT = 4 the ring slot of every plane a step touches
 // is a compile-time function of the phase, so no LDS address is recomputed per step.
+// WIN: every stencil read of one field in one ring plane goes through one LDS base register
+// (the window's first cell, pinned in a VGPR), so that all its reads pair into ds_read2 with
+// immediate offsets; otherwise the compiler forms a new base (one VALU add) for most pairs.
 template <int TX, int TY, bool COMP = false, int NSLOT = 4, bool DEFER = false,
-          bool PREV_FIRST = false, int NQ = 7>
+          bool PREV_FIRST = false, int NQ = 7, bool WIN = false>
 struct ZMarchLean {
     static_assert(NQ == 7 || (NQ == 8 && NSLOT == 4), "8-phase loop needs the 4-slot ring");
     static constexpr int NT  = TX * TY;
@@ -711,14 +714,43 @@ struct ZMarchLean {
     }
 
     // Value and z derivatives of field f at plane z: qv (z-3..z-1) and ring slots z..z+3.
-    __device__ __forceinline__ void zderivs(const State& st, int f, const AcReal* r0,
-                                            const AcReal* r1, const AcReal* r2, const AcReal* r3,
+    // w: field f's cell in plane z; c1, c2, c3: this thread's cell of planes z+1..z+3 (field 0)
+    template <typename PW, typename PC>
+    __device__ __forceinline__ void zderivs(const State& st, int f, PW w, PC c1, PC c2, PC c3,
                                             AcReal& v, AcReal& dz, AcReal& dzz) const
     {
-        const int o        = f * LNP + cofs;
-        const AcReal zq[7] = {st.qv[f][0], st.qv[f][1], st.qv[f][2], r0[o], r1[o], r2[o], r3[o]};
+        const int o        = f * LNP;
+        const AcReal zq[7] = {st.qv[f][0], st.qv[f][1], st.qv[f][2], w[0], c1[o], c2[o], c3[o]};
         v                  = zq[3];
         axis_queue(zq, a.phys.d1[2], a.phys.d2[2], dz, dzz);
+    }
+
+    typedef const AcReal __attribute__((address_space(3))) LdsReal;
+    static constexpr int W0 = 3 * LX + 3; // window corner (-3, -3) to its centre
+
+    // LDS address laundered through a VGPR: the compiler cannot rebase reads made through it.
+    static __device__ __forceinline__ const LdsReal* lds_pin(const AcReal* q)
+    {
+        unsigned addr = (unsigned)(size_t)(const LdsReal*)q;
+        asm volatile("" : "+v"(addr));
+        return (const LdsReal*)(size_t)addr;
+    }
+
+    // This thread's cell of field plane pf (pf: first cell of a field in a ring plane).
+    __device__ __forceinline__ auto centre(const AcReal* pf) const
+    {
+        if constexpr (WIN)
+            return lds_pin(pf + cofs - W0) + W0;
+        else
+            return pf + cofs;
+    }
+    // This thread's cell of field 0 of ring plane rp (other fields at multiples of LNP).
+    __device__ __forceinline__ auto plane_cell(const AcReal* rp) const
+    {
+        if constexpr (WIN)
+            return lds_pin(rp + cofs);
+        else
+            return rp + cofs;
     }
 
     template <int PH, bool OUT> __device__ __forceinline__ void step(State& st, int p) const
@@ -770,12 +802,12 @@ struct ZMarchLean {
         for (int i = 1; i <= 3; ++i) {
             const int lo = (PH - i + NQ) % NQ, hi = (PH + i) % NQ; // outputs p - i and p + i
             const AcReal cx = P.cxz[i], cy = P.cyz[i];
-            const AcReal* ux = rs + VTXBUF_UUX * LNP + cofs;
-            const AcReal* uy = rs + VTXBUF_UUY * LNP + cofs;
-            const AcReal* uz = rs + VTXBUF_UUZ * LNP + cofs;
-            const AcReal* ax = rs + VTXBUF_AX * LNP + cofs;
-            const AcReal* ay = rs + VTXBUF_AY * LNP + cofs;
-            const AcReal* az = rs + VTXBUF_AZ * LNP + cofs;
+            const auto ux = centre(rs + VTXBUF_UUX * LNP);
+            const auto uy = centre(rs + VTXBUF_UUY * LNP);
+            const auto uz = centre(rs + VTXBUF_UUZ * LNP);
+            const auto ax = centre(rs + VTXBUF_AX * LNP);
+            const auto ay = centre(rs + VTXBUF_AY * LNP);
+            const auto az = centre(rs + VTXBUF_AZ * LNP);
             const AcReal d0 = cx * (uz[i] - uz[-i]);
             const AcReal d1 = cy * (uz[i * LX] - uz[-i * LX]);
             const AcReal d2 = cx * (ux[i] - ux[-i]) + cy * (uy[i * LX] - uy[-i * LX]);
@@ -820,59 +852,68 @@ struct ZMarchLean {
             &sh.ring[NQ == 8 ? ((PH + 3) & 3) : ring_slot(p + NSLOT - 1)][0][0]; // plane z + 2
         MhdAgg g;
         AcReal v, dx, dy, dz, dxx, dyy, dzz;
+        const auto c1 = plane_cell(r1), c2 = plane_cell(r2), c3 = plane_cell(rs);
 
-        zderivs(st, VTXBUF_LNRHO, ro, r1, r2, rs, v, dz, dzz);
-        inplane_lds<LX>(ro + VTXBUF_LNRHO * LNP, cofs, P, v, dx, dy, dxx, dyy);
+        const auto wlnrho = centre(ro + VTXBUF_LNRHO * LNP);
+        zderivs(st, VTXBUF_LNRHO, wlnrho, c1, c2, c3, v, dz, dzz);
+        inplane_lds<LX>(wlnrho, 0, P, v, dx, dy, dxx, dyy);
         g.lr_v = v, g.lr_x = dx, g.lr_y = dy, g.lr_z = dz, g.lr_lap = dxx + dyy + dzz;
 
-        zderivs(st, VTXBUF_ENTROPY, ro, r1, r2, rs, v, dz, dzz);
-        inplane_lds<LX>(ro + VTXBUF_ENTROPY * LNP, cofs, P, v, dx, dy, dxx, dyy);
+        const auto wentropy = centre(ro + VTXBUF_ENTROPY * LNP);
+        zderivs(st, VTXBUF_ENTROPY, wentropy, c1, c2, c3, v, dz, dzz);
+        inplane_lds<LX>(wentropy, 0, P, v, dx, dy, dxx, dyy);
         g.s_v = v, g.s_x = dx, g.s_y = dy, g.s_z = dz, g.s_lap = dxx + dyy + dzz;
 
         // grad(div u)_r = sum_c d_r d_c u_c; the x-z / y-z parts come from x[0..2]
-        zderivs(st, VTXBUF_UUX, ro, r1, r2, rs, v, dz, dzz);
-        inplane_lds<LX>(ro + VTXBUF_UUX * LNP, cofs, P, v, dx, dy, dxx, dyy);
-        const AcReal ux_xy = cross_xy_lds<LX>(ro + VTXBUF_UUX * LNP, cofs, P.cxy);
+        const auto wuux = centre(ro + VTXBUF_UUX * LNP);
+        zderivs(st, VTXBUF_UUX, wuux, c1, c2, c3, v, dz, dzz);
+        inplane_lds<LX>(wuux, 0, P, v, dx, dy, dxx, dyy);
+        const AcReal ux_xy = cross_xy_lds<LX>(wuux, 0, P.cxy);
         g.uv[0] = v, g.ug[0][0] = dx, g.ug[0][1] = dy, g.ug[0][2] = dz;
         g.ulap[0] = dxx + dyy + dzz;
         g.gdu[0]  = dxx + st.x[0][OS];
         g.gdu[1]  = ux_xy + st.x[1][OS];
 
-        zderivs(st, VTXBUF_UUY, ro, r1, r2, rs, v, dz, dzz);
-        inplane_lds<LX>(ro + VTXBUF_UUY * LNP, cofs, P, v, dx, dy, dxx, dyy);
-        const AcReal uy_xy = cross_xy_lds<LX>(ro + VTXBUF_UUY * LNP, cofs, P.cxy);
+        const auto wuuy = centre(ro + VTXBUF_UUY * LNP);
+        zderivs(st, VTXBUF_UUY, wuuy, c1, c2, c3, v, dz, dzz);
+        inplane_lds<LX>(wuuy, 0, P, v, dx, dy, dxx, dyy);
+        const AcReal uy_xy = cross_xy_lds<LX>(wuuy, 0, P.cxy);
         g.uv[1] = v, g.ug[1][0] = dx, g.ug[1][1] = dy, g.ug[1][2] = dz;
         g.ulap[1] = dxx + dyy + dzz;
         g.gdu[0] += uy_xy;
         g.gdu[1] += dyy;
 
-        zderivs(st, VTXBUF_UUZ, ro, r1, r2, rs, v, dz, dzz);
-        inplane_lds<LX>(ro + VTXBUF_UUZ * LNP, cofs, P, v, dx, dy, dxx, dyy);
+        const auto wuuz = centre(ro + VTXBUF_UUZ * LNP);
+        zderivs(st, VTXBUF_UUZ, wuuz, c1, c2, c3, v, dz, dzz);
+        inplane_lds<LX>(wuuz, 0, P, v, dx, dy, dxx, dyy);
         g.uv[2] = v, g.ug[2][0] = dx, g.ug[2][1] = dy, g.ug[2][2] = dz;
         g.ulap[2] = dxx + dyy + dzz;
         g.gdu[2]  = st.x[2][OS] + dzz;
 
         // grad(div A) - lap A; the own-axis second derivative cancels
-        zderivs(st, VTXBUF_AX, ro, r1, r2, rs, v, dz, dzz);
-        inplane_lds<LX>(ro + VTXBUF_AX * LNP, cofs, P, v, dx, dy, dxx, dyy);
-        const AcReal ax_xy = cross_xy_lds<LX>(ro + VTXBUF_AX * LNP, cofs, P.cxy);
+        const auto wax = centre(ro + VTXBUF_AX * LNP);
+        zderivs(st, VTXBUF_AX, wax, c1, c2, c3, v, dz, dzz);
+        inplane_lds<LX>(wax, 0, P, v, dx, dy, dxx, dyy);
+        const AcReal ax_xy = cross_xy_lds<LX>(wax, 0, P.cxy);
         g.alap[0] = dxx + dyy + dzz;
         g.jj[0]   = st.x[3][OS] - (dyy + dzz);
         g.jj[1]   = ax_xy + st.x[4][OS];
         g.B[1]    = dz;
         g.B[2]    = -dy;
 
-        zderivs(st, VTXBUF_AY, ro, r1, r2, rs, v, dz, dzz);
-        inplane_lds<LX>(ro + VTXBUF_AY * LNP, cofs, P, v, dx, dy, dxx, dyy);
-        const AcReal ay_xy = cross_xy_lds<LX>(ro + VTXBUF_AY * LNP, cofs, P.cxy);
+        const auto way = centre(ro + VTXBUF_AY * LNP);
+        zderivs(st, VTXBUF_AY, way, c1, c2, c3, v, dz, dzz);
+        inplane_lds<LX>(way, 0, P, v, dx, dy, dxx, dyy);
+        const AcReal ay_xy = cross_xy_lds<LX>(way, 0, P.cxy);
         g.alap[1] = dxx + dyy + dzz;
         g.jj[0] += ay_xy;
         g.jj[1] -= dxx + dzz;
         g.B[0] = -dz;
         g.B[2] += dx;
 
-        zderivs(st, VTXBUF_AZ, ro, r1, r2, rs, v, dz, dzz);
-        inplane_lds<LX>(ro + VTXBUF_AZ * LNP, cofs, P, v, dx, dy, dxx, dyy);
+        const auto waz = centre(ro + VTXBUF_AZ * LNP);
+        zderivs(st, VTXBUF_AZ, waz, c1, c2, c3, v, dz, dzz);
+        inplane_lds<LX>(waz, 0, P, v, dx, dy, dxx, dyy);
         g.alap[2] = dxx + dyy + dzz;
         g.jj[2]   = st.x[5][OS] - (dxx + dyy);
         g.B[0] += dy;
@@ -1329,13 +1370,13 @@ strip_tile(const int b, const int ntx, const int nty, int& btx, int& bty, int& b
 
 // One z-march column: tile (btx, bty) of box `a`, output planes [zs, min(zs + zdepth, end.z)).
 template <int TX, int TY, bool COMP = false, int NSLOT = 4, bool DEFER = false,
-          bool PREV_FIRST = false, int NQ = 7>
+          bool PREV_FIRST = false, int NQ = 7, bool WIN = false>
 __device__ __forceinline__ void
 zlean_column(const SubstepArgs& a,
-             typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ>::Shared& sh,
+             typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN>::Shared& sh,
              const int btx, const int bty, const int zs, const int zdepth)
 {
-    typedef ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ> K;
+    typedef ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN> K;
     K k(a, sh);
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = lean_tile_row<TX, TY>(tid);
@@ -1412,32 +1453,34 @@ zlean_column(const SubstepArgs& a,
 }
 
 template <int TX, int TY, int W, bool COMP = false, int NSLOT = 4, bool DEFER = false,
-          bool PREV_FIRST = false, int NQ = 7>
+          bool PREV_FIRST = false, int NQ = 7, bool WIN = false>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W)))
 solve_zlean(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
 {
-    __shared__ typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ>::Shared sh;
+    __shared__ typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN>::Shared sh;
     int btx, bty, btz;
     strip_tile(xcd_remap((int)blockIdx.x, (int)gridDim.x), ntx, nty, btx, bty, btz);
-    zlean_column<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ>(a, sh, btx, bty,
-                                                             a.start.z + btz * zdepth, zdepth);
+    zlean_column<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN>(a, sh, btx, bty,
+                                                                  a.start.z + btz * zdepth, zdepth);
 }
 
 // Persistent form for ensembles of independent boxes: a grid sized to the resident block slots
 // walks a (box, tile column, z range) work list; each box brings its own arguments (pointers,
 // sizes, physical parameters), read from global memory.
-// The 16 x 16 build takes the single-box FP32 default's options (5-slot ring, previous stage
-// first, deferred stores, two waves per SIMD).
-template <int TX, int TY, int W = 1, int NSLOT = 4, bool DEFER = false, bool PREV_FIRST = false>
+// The 16 x 16 FP32 build uses a 5-slot ring, previous stage first, deferred stores and two
+// waves per SIMD: on the ensemble's short columns (32-128 planes) it measured 11.4 ms per
+// 96-box step against 11.8 ms for the single-box default's options (pinned LDS windows).
+template <int TX, int TY, int W = 1, int NSLOT = 4, bool DEFER = false, bool PREV_FIRST = false,
+          bool WIN = false>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W)))
 solve_zlean_batch(const SubstepArgs* __restrict__ boxes, const BatchItem* __restrict__ items,
                   const int nitems)
 {
-    __shared__ typename ZMarchLean<TX, TY, false, NSLOT, DEFER, PREV_FIRST>::Shared sh;
+    __shared__ typename ZMarchLean<TX, TY, false, NSLOT, DEFER, PREV_FIRST, 7, WIN>::Shared sh;
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const BatchItem w = items[it];
-        zlean_column<TX, TY, false, NSLOT, DEFER, PREV_FIRST>(boxes[w.box], sh, w.btx, w.bty, w.zs,
-                                                              w.zdepth);
+        zlean_column<TX, TY, false, NSLOT, DEFER, PREV_FIRST, 7, WIN>(boxes[w.box], sh, w.btx,
+                                                                      w.bty, w.zs, w.zdepth);
     }
 }
 
@@ -1491,6 +1534,9 @@ constexpr ConfigEntry kConfigs[] = {
     {12, 16, 16},
     {13, 16, 16},
     {16, 16, 16},
+    {17, 16, 16},
+    {19, 16, 16},
+    {22, 16, 16},
 #endif
     {8, 16, 16},
 };
@@ -1560,15 +1606,15 @@ kernel_config_at(int index)
 AcKernelConfig
 default_kernel_config(int nx, int ny, int nz)
 {
-    // Fastest at 512^3: the lean z-march that loads the previous stage first and defers its
-    // stores, on 16 x 16 tiles; in FP32 with a 5-slot ring (one barrier per plane) compiled for
-    // two waves per SIMD (variant 16: two blocks of 4 waves per CU, 80 KiB of LDS each), in FP64
-    // with a 4-slot ring for one wave per SIMD (variant 8: its 128 KiB ring leaves room for one
-    // block per CU). The z extent of a block is cut so that the grid still has about four blocks
-    // per concurrently resident block slot, in multiples of the unrolled 7-plane loop.
+    // Fastest at 512^3: the lean z-march on 16 x 16 tiles with a 4-slot ring, loading the
+    // previous stage first and deferring its stores; in FP32 compiled for two waves per SIMD with
+    // pinned LDS windows (variant 19: two blocks of 4 waves per CU, 64 KiB of LDS each), in FP64
+    // for one wave per SIMD (variant 8: its 128 KiB ring leaves room for one block per CU). The
+    // z extent of a block is cut so that the grid still has about four blocks per concurrently
+    // resident block slot, in multiples of the unrolled 7-plane loop.
     AcKernelConfig c;
 #if AC_DOUBLE_PRECISION == 0
-    c.variant        = 16;
+    c.variant        = 19;
     c.tile_x         = 16;
     c.tile_y         = 16;
     const long want  = 1024;
@@ -1673,10 +1719,21 @@ launch_substep(const AcKernelConfig& c, const SubstepArgs& a, hipStream_t stream
     if (c.variant == 13 && c.tile_x == 16 && c.tile_y == 16)
         return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2, false, 4, true, true, 8>, a,
                                           c.z_depth, stream);
-    // 5-slot ring (one barrier per plane), two waves per SIMD: 77 KiB of LDS per block
+    // 5-slot ring (one barrier per plane), two waves per SIMD: 80 KiB of LDS per block
     if (c.variant == 16 && c.tile_x == 16 && c.tile_y == 16)
         return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2, false, 5, true, true>, a,
                                           c.z_depth, stream);
+    // pinned LDS windows (WIN) on the two-wave builds: 5-slot ring; 4-slot ring with deferred
+    // stores and previous stage first; 4-slot ring with deferred stores
+    if (c.variant == 17 && c.tile_x == 16 && c.tile_y == 16)
+        return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2, false, 5, true, true, 7, true>,
+                                          a, c.z_depth, stream);
+    if (c.variant == 19 && c.tile_x == 16 && c.tile_y == 16)
+        return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2, false, 4, true, true, 7, true>,
+                                          a, c.z_depth, stream);
+    if (c.variant == 22 && c.tile_x == 16 && c.tile_y == 16)
+        return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2, false, 4, true, false, 7, true>,
+                                          a, c.z_depth, stream);
 #endif
     if (c.variant == 8 && c.tile_x == 16 && c.tile_y == 16)
         return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 1, false, 4, true, true>, a,

@@ -55,7 +55,9 @@ AcResult acDeviceIntegrateStepGraph(const Device device, const Stream stream, co
  * variants 10-12: two-waves-per-SIMD builds of the lean z-march on 16 x 16 tiles (10: deferred
  *            stores, 11: previous stage first, 12: both).
  * variant 13: variant 12 with an 8-plane loop, whose LDS ring slots are compile-time (FP32).
- * variant 16: variant 12 with the 5-slot ring (one barrier per plane; the FP32 default). */
+ * variant 16: variant 12 with the 5-slot ring (one barrier per plane).
+ * variants 17, 19, 22: variants 16, 12 and 10 reading each field's LDS stencil window through
+ *            one pinned base register (FP32; variant 19 is the FP32 default). */
 typedef struct {
     int variant;
     int tile_x;
