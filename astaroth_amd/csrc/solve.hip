@@ -793,6 +793,16 @@ struct ZMarchLean {
             for (int f = 0; f < NF; ++f)
                 prev[f] = ld(a.out[f] + oz, ow);
         }
+        // compensated integration: the rounding-error terms of the output cell, loaded with the
+        // previous stage so that their latency is hidden behind the step's arithmetic
+        AcReal cin[COMP ? NF : 1], cprev[COMP ? NF : 1];
+        if (COMP && OUT) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                cin[f]   = ld(a.cin[f] + oz, ow);
+                cprev[f] = a.rk.step > 0 ? ld(a.cout[f] + oz, ow) : AcReal(0);
+            }
+        }
         if (PREV_FIRST)
             issue_loads(st, NSLOT == 4 ? p + 1 : p + 2);
 
@@ -923,7 +933,7 @@ struct ZMarchLean {
         mhd_rates_agg(g, P, rate);
         push_queue(st, ro); // qv[f][2] is now plane z
         if (COMP) {
-            store_compensated(st, rate, prev, oz, ow, z);
+            store_compensated(st, rate, prev, cin, cprev, oz, ow, z);
             return;
         }
         if (a.rk.step == 0) {
@@ -955,7 +965,8 @@ struct ZMarchLean {
     //   u_out + c_out = u_in + d exactly.
     // The stencils read u only.
     __device__ __forceinline__ void store_compensated(const State& st, const AcReal* rate,
-                                                      const AcReal* prev, unsigned long oz,
+                                                      const AcReal* prev, const AcReal* cin,
+                                                      const AcReal* cprev, unsigned long oz,
                                                       unsigned ow, int z) const
     {
         if (!(z < ze && active))
@@ -963,13 +974,13 @@ struct ZMarchLean {
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
             const AcReal u  = st.qv[f][2];
-            const AcReal ci = ld(a.cin[f] + oz, ow);
+            const AcReal ci = cin[f];
             AcReal d;
             if (a.rk.step == 0) {
                 d = a.rk.beta_dt * rate[f] + ci;
             }
             else {
-                const AcReal cp = ld(a.cout[f] + oz, ow);
+                const AcReal cp = cprev[f];
                 d = a.rk.beta * (a.rk.ab * ((u - prev[f]) + (ci - cp)) + a.rk.dt * rate[f]) + ci;
             }
             const AcReal sum = u + d;
@@ -1645,7 +1656,8 @@ launch_substep(const AcKernelConfig& c, const SubstepArgs& a, hipStream_t stream
         // compensated integration: the lean z-march in its default tile, z-depth from `c`
         const int zd = c.variant >= 1 ? c.z_depth : 0;
 #if AC_DOUBLE_PRECISION == 0
-        return launch_zmarch_grid<32, 4>(solve_zlean<32, 4, 1, true>, a, zd, stream);
+        return launch_zmarch_grid<32, 4>(solve_zlean<32, 4, 1, true, 4, false, true>, a, zd,
+                                         stream);
 #else
         return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 1, true>, a, zd, stream);
 #endif
