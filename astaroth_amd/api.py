@@ -157,6 +157,7 @@ class Library:
             "acHostMeshDestroy": (i, [ctypes.POINTER(M)]),
             "acLoadConfig": (i, [ctypes.c_char_p, ctypes.POINTER(MI)]),
             "acDeviceGetMeshInfo": (i, [vp, ctypes.POINTER(MI)]),
+            "acDeviceDiagnostics": (i, [vp, i, ctypes.POINTER(ctypes.c_double)]),
             "acDeviceIntegrateStepGBC": (i, [vp, i, MI, R]),
             "acHostWriteMeshInfo": (i, [MI, ctypes.c_char_p]),
             "acHostSaveMeshSnapshot": (i, [M, i, R, ctypes.c_char_p]),
@@ -429,6 +430,13 @@ class Device:
         out = ctypes.c_double()
         self._check(self.lib.dll.acDeviceTimestepCFL(self.handle, stream, ctypes.byref(out)), "acDeviceTimestepCFL")
         return out.value
+
+    def diagnostics(self, stream=abi.STREAM_DEFAULT) -> dict:
+        """acDeviceDiagnostics: {'uu': [max, min, rms], 'aa': [...], 'lnrho': [...]} (FP64)."""
+        out = (ctypes.c_double * 9)()
+        self._check(self.lib.dll.acDeviceDiagnostics(self.handle, stream, out), "acDeviceDiagnostics")
+        v = list(out)
+        return {"uu": v[0:3], "aa": v[3:6], "lnrho": v[6:9]}
 
     def mesh_info(self):
         info = self.lib.MeshInfo()

@@ -114,6 +114,11 @@ hipError_t launch_general_fill(AcReal* const* fields, int nfields, int nx, int n
 
 // Reductions (reductions.hip). `vals` are 1, 3 or 4 field pointers. Writes the FP64 result to
 // *d_result (device) and returns the launch status.
+// One-pass diagnostics of the `in` fields: max, min and sum of squares of |u|, |A| and lnrho
+// over the interior, FP64, into d_result[9] (reductions.hip).
+int diagnostics_partials_needed();
+hipError_t launch_diagnostics(const AcReal* const* fields, const AcMeshInfo& info,
+                              double* d_partials, double* d_result, hipStream_t stream);
 hipError_t launch_reduce(int rtype, int nvals, const AcReal* const* vals, const AcMeshInfo& info,
                          double* d_partials, int max_partials, double* d_result,
                          hipStream_t stream);
@@ -150,6 +155,9 @@ struct device_s {
     double* reduce_partials;
     double* reduce_result;
     double* reduce_result_host; // pinned
+    double* diag_partials;      // acDeviceDiagnostics: per-block partials, results, pinned copy
+    double* diag_result;
+    double* diag_result_host;
     AcKernelConfig kernel_config;
     // compensated integration (acDeviceSetCompensatedIntegration): comp[0] pairs with vba.in,
     // comp[1] with vba.out; null when off

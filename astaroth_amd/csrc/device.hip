@@ -468,6 +468,9 @@ acDeviceCreate(const int id, const AcMeshInfo config, Device* device_handle)
     AC_HIP_CHECK(hipMalloc((void**)&d->reduce_partials, sizeof(double) * reduce_partials_needed()));
     AC_HIP_CHECK(hipMalloc((void**)&d->reduce_result, sizeof(double)));
     AC_HIP_CHECK(hipHostMalloc((void**)&d->reduce_result_host, sizeof(double), hipHostMallocDefault));
+    AC_HIP_CHECK(hipMalloc((void**)&d->diag_partials, sizeof(double) * diagnostics_partials_needed()));
+    AC_HIP_CHECK(hipMalloc((void**)&d->diag_result, 9 * sizeof(double)));
+    AC_HIP_CHECK(hipHostMalloc((void**)&d->diag_result_host, 9 * sizeof(double), hipHostMallocDefault));
     AC_HIP_CHECK(hipStreamSynchronize(d->streams[0]));
 
     acDeviceLoadDefaultUniforms(d);
@@ -494,6 +497,9 @@ acDeviceDestroy(Device device)
     AC_HIP_CHECK(hipFree(device->reduce_partials));
     AC_HIP_CHECK(hipFree(device->reduce_result));
     AC_HIP_CHECK(hipHostFree(device->reduce_result_host));
+    AC_HIP_CHECK(hipFree(device->diag_partials));
+    AC_HIP_CHECK(hipFree(device->diag_result));
+    AC_HIP_CHECK(hipHostFree(device->diag_result_host));
     if (device->halo_buffer)
         AC_HIP_CHECK(hipFree(device->halo_buffer));
     for (int set = 0; set < 2; ++set)
@@ -1254,6 +1260,27 @@ acDeviceGetStreamHandle(const Device device, const Stream stream)
     if (!stream_ok(device, stream))
         return nullptr;
     return (void*)device->streams[stream];
+}
+
+AcResult
+acDeviceDiagnostics(const Device device, const Stream stream, double* out)
+{
+    if (!stream_ok(device, stream) || !out)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(device->id));
+    hipStream_t s = device->streams[stream];
+    const AcReal* f[NF];
+    for (int q = 0; q < NF; ++q)
+        f[q] = device->vba.in[q];
+    AC_HIP_CHECK(launch_diagnostics(f, device->local_config, device->diag_partials,
+                                    device->diag_result, s));
+    AC_HIP_CHECK(hipMemcpyAsync(device->diag_result_host, device->diag_result, 9 * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+    AC_HIP_CHECK(hipStreamSynchronize(s));
+    const double n = (double)nx_of(device) * ny_of(device) * nz_of(device);
+    for (int i = 0; i < 9; ++i)
+        out[i] = i % 3 == 2 ? std::sqrt(device->diag_result_host[i] / n) : device->diag_result_host[i];
+    return AC_SUCCESS;
 }
 
 AcResult

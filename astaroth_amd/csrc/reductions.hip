@@ -165,7 +165,127 @@ reduce_partials(const double* partials, int n, int op, double* result)
         *result = r;
 }
 
+// Diagnostics in one pass: max, min and sum of squares of |u|, |A| and lnrho over the interior,
+// with the filters of the single reductions (MAX/MIN of a vector length rounded in AcReal as in
+// filter<3, RTYPE_MAX>, sums of squares in FP64), so that the maxima and minima equal theirs
+// bit for bit. Each lane loads U cells of its row (7 fields each) before folding them.
+constexpr int NSTAT = 9; // u max, min, sum2; A max, min, sum2; lnrho max, min, sum2
+
+struct DiagArgs {
+    const AcReal* f[NF];
+    int nx, ny, nz, mx, mxy;
+};
+
+template <int U>
+__global__ void __launch_bounds__(RED_THREADS)
+diagnostics_rows(const DiagArgs a, double* partials)
+{
+    __shared__ double lds[NSTAT][RED_WAVES];
+    const int lane  = threadIdx.x & 63;
+    const int gwave = blockIdx.x * RED_WAVES + (threadIdx.x >> 6);
+    const int nwave = gridDim.x * RED_WAVES;
+    const int nrows = a.ny * a.nz;
+    double v[NSTAT] = {-INFINITY, INFINITY, 0.0, -INFINITY, INFINITY, 0.0, -INFINITY, INFINITY, 0.0};
+    for (int r = gwave; r < nrows; r += nwave) {
+        const int j = 3 + r % a.ny, k = 3 + r / a.ny;
+        const long row = (long)j * a.mx + (long)k * a.mxy + 3;
+        for (int i0 = lane; i0 < a.nx; i0 += 64 * U) {
+            AcReal u[U][3], b[U][3], l[U];
+#pragma unroll
+            for (int w = 0; w < U; ++w) {
+                const int i   = i0 + 64 * w;
+                const bool ok = i < a.nx;
+                u[w][0] = ok ? a.f[VTXBUF_UUX][row + i] : AcReal(0);
+                u[w][1] = ok ? a.f[VTXBUF_UUY][row + i] : AcReal(0);
+                u[w][2] = ok ? a.f[VTXBUF_UUZ][row + i] : AcReal(0);
+                b[w][0] = ok ? a.f[VTXBUF_AX][row + i] : AcReal(0);
+                b[w][1] = ok ? a.f[VTXBUF_AY][row + i] : AcReal(0);
+                b[w][2] = ok ? a.f[VTXBUF_AZ][row + i] : AcReal(0);
+                l[w]    = ok ? a.f[VTXBUF_LNRHO][row + i] : AcReal(0);
+            }
+#pragma unroll
+            for (int w = 0; w < U; ++w) {
+                if (i0 + 64 * w >= a.nx)
+                    continue;
+                const double un = filter<3, RTYPE_MAX>(u[w]), us = filter<3, RTYPE_RMS>(u[w]);
+                const double an = filter<3, RTYPE_MAX>(b[w]), as = filter<3, RTYPE_RMS>(b[w]);
+                const double ln = filter<1, RTYPE_MAX>(&l[w]), ls = filter<1, RTYPE_RMS>(&l[w]);
+                v[0] = combine(OP_MAX, v[0], un);
+                v[1] = combine(OP_MIN, v[1], un);
+                v[2] += us;
+                v[3] = combine(OP_MAX, v[3], an);
+                v[4] = combine(OP_MIN, v[4], an);
+                v[5] += as;
+                v[6] = combine(OP_MAX, v[6], ln);
+                v[7] = combine(OP_MIN, v[7], ln);
+                v[8] += ls;
+            }
+        }
+    }
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NSTAT; ++q) {
+        const int op = q % 3 == 0 ? OP_MAX : q % 3 == 1 ? OP_MIN : OP_SUM;
+        const double x = wave_reduce(op, v[q]);
+        if (lane == 0)
+            lds[q][w] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NSTAT) {
+        const int q  = threadIdx.x;
+        const int op = q % 3 == 0 ? OP_MAX : q % 3 == 1 ? OP_MIN : OP_SUM;
+        double x     = lds[q][0];
+        for (int i = 1; i < RED_WAVES; ++i)
+            x = combine(op, x, lds[q][i]);
+        partials[(long)blockIdx.x * NSTAT + q] = x;
+    }
+}
+
+// Block q folds statistic q of every block's partials in a fixed order.
+__global__ void __launch_bounds__(RED_THREADS)
+diagnostics_finalize(const double* partials, int nblocks, double* result)
+{
+    __shared__ double lds[RED_WAVES];
+    const int q  = blockIdx.x;
+    const int op = q % 3 == 0 ? OP_MAX : q % 3 == 1 ? OP_MIN : OP_SUM;
+    double acc   = identity(op);
+    for (int i = threadIdx.x; i < nblocks; i += RED_THREADS)
+        acc = combine(op, acc, partials[(long)i * NSTAT + q]);
+    const double r = block_reduce(op, acc, lds);
+    if (threadIdx.x == 0)
+        result[q] = r;
+}
+
 } // namespace
+
+int
+diagnostics_partials_needed()
+{
+    return MAX_PARTIALS * NSTAT;
+}
+
+hipError_t
+launch_diagnostics(const AcReal* const* fields, const AcMeshInfo& info, double* d_partials,
+                   double* d_result, hipStream_t stream)
+{
+    DiagArgs a;
+    for (int q = 0; q < NF; ++q)
+        a.f[q] = fields[q];
+    a.nx   = info.int_params[AC_nx];
+    a.ny   = info.int_params[AC_ny];
+    a.nz   = info.int_params[AC_nz];
+    a.mx   = a.nx + 6;
+    a.mxy  = a.mx * (a.ny + 6);
+    int blocks = (a.ny * a.nz + RED_WAVES * 4 - 1) / (RED_WAVES * 4);
+    blocks     = blocks < 1 ? 1 : blocks > MAX_PARTIALS ? MAX_PARTIALS : blocks;
+    diagnostics_rows<4><<<dim3(blocks), dim3(RED_THREADS), 0, stream>>>(a, d_partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    diagnostics_finalize<<<dim3(NSTAT), dim3(RED_THREADS), 0, stream>>>(d_partials, blocks,
+                                                                        d_result);
+    return hipGetLastError();
+}
 
 int
 reduce_partials_needed()

@@ -129,21 +129,9 @@ def run_gpu(double: bool, n: int, steps: int, warmup: int, dt: float, autotune: 
 
 def step_diagnostics(dev):
     """FP64 max/min/rms of |u|, |A| and lnrho over the interior (RMS finalised as sqrt(sum/N),
-    as the reference's node layer does, node.cc:932-957)."""
-    n3 = float(dev.info.int_params[abi_ip("AC_nxyz")])
-    out = {}
-    for name, args in (("uu", ("VTXBUF_UUX", "VTXBUF_UUY", "VTXBUF_UUZ")),
-                       ("aa", ("VTXBUF_AX", "VTXBUF_AY", "VTXBUF_AZ"))):
-        out[name] = [dev.reduce_vec(r, *args, double=True) for r in ("RTYPE_MAX", "RTYPE_MIN")]
-        out[name].append((dev.reduce_vec("RTYPE_RMS", *args, double=True) / n3) ** 0.5)
-    out["lnrho"] = [dev.reduce_scal(r, "VTXBUF_LNRHO", double=True) for r in ("RTYPE_MAX", "RTYPE_MIN")]
-    out["lnrho"].append((dev.reduce_scal("RTYPE_RMS", "VTXBUF_LNRHO", double=True) / n3) ** 0.5)
-    return out
-
-
-def abi_ip(name):
-    from astaroth_amd.abi import IP
-    return IP[name]
+    as the reference's node layer does, node.cc:932-957), in one pass (acDeviceDiagnostics;
+    the same values as nine acDeviceReduce*Double calls, read once)."""
+    return dev.diagnostics()
 
 
 def precision_study(n: int, steps: int, dt: float, hip: Hip):

@@ -178,6 +178,13 @@ void* acEnsembleGetStreamHandle(const Ensemble ensemble);
 /* ---- Stream access for timing from the caller (returns the hipStream_t as void*) ---- */
 void* acDeviceGetStreamHandle(const Device device, const Stream stream);
 
+/* Diagnostics of the `in` state in one pass over the interior (synchronous), FP64:
+ * out[0..2] = max, min, rms of |u|; out[3..5] = the same of |A|; out[6..8] = max, min, rms of
+ * lnrho (rms = sqrt(sum of squares / (nx ny nz))). The maxima and minima equal those of
+ * acDeviceReduceVecDouble / acDeviceReduceScalDouble bit for bit; the sums differ from theirs
+ * only by summation order. One read of 7 fields instead of 21 for the nine single reductions. */
+AcResult acDeviceDiagnostics(const Device device, const Stream stream, double* out);
+
 /* The parameters in effect on the Device (its creation configuration with every uniform loaded
  * since). */
 AcResult acDeviceGetMeshInfo(const Device device, AcMeshInfo* info);

@@ -497,3 +497,35 @@ def test_cfl_timestep(lib):
     uu_dt = rp("AC_cdt") * dsmin / (umax + math.sqrt(rp("AC_cs2_sound")))
     visc_dt = rp("AC_cdtv") * dsmin ** 2 / max(max(rp("AC_nu_visc"), rp("AC_eta")), rp("AC_gamma"))
     assert dt == pytest.approx(min(uu_dt, visc_dt), rel=1e-12)
+
+
+@pytest.mark.parametrize("dims", [(40, 36, 20), (17, 9, 23)])
+def test_one_pass_diagnostics_match_single_reductions(lib, dims):
+    """acDeviceDiagnostics: maxima and minima bitwise equal to the single FP64 reductions (same
+    filters), rms within summation-order rounding of them and of numpy on the stored state."""
+    nx, ny, nz = dims
+    info = lib.config(n=nx, ny=ny, nz=nz)
+    fields = mhd_fields(nx, ny, nz, np.float64, config_id=5)
+    with make_device(lib, info) as dev:
+        dev.load_mesh(host_mesh_from(lib, info, fields))
+        dev.integrate_step(1e-3)
+        d = dev.diagnostics()
+        n3 = float(nx * ny * nz)
+        for name, args in (("uu", ("VTXBUF_UUX", "VTXBUF_UUY", "VTXBUF_UUZ")),
+                           ("aa", ("VTXBUF_AX", "VTXBUF_AY", "VTXBUF_AZ"))):
+            assert d[name][0] == dev.reduce_vec("RTYPE_MAX", *args, double=True)
+            assert d[name][1] == dev.reduce_vec("RTYPE_MIN", *args, double=True)
+            rms = (dev.reduce_vec("RTYPE_RMS", *args, double=True) / n3) ** 0.5
+            assert d[name][2] == pytest.approx(rms, rel=1e-13)
+        assert d["lnrho"][0] == dev.reduce_scal("RTYPE_MAX", "VTXBUF_LNRHO", double=True)
+        assert d["lnrho"][1] == dev.reduce_scal("RTYPE_MIN", "VTXBUF_LNRHO", double=True)
+        rms = (dev.reduce_scal("RTYPE_RMS", "VTXBUF_LNRHO", double=True) / n3) ** 0.5
+        assert d["lnrho"][2] == pytest.approx(rms, rel=1e-13)
+        out = host_mesh_from(lib, info, fields)
+        dev.store_mesh(out)
+    inner = [out.fields[q][3:-3, 3:-3, 3:-3].astype(np.float64) for q in range(8)]
+    lr = inner[abi.VTX["VTXBUF_LNRHO"]]
+    assert d["lnrho"][0] == lr.max() and d["lnrho"][1] == lr.min()
+    assert d["lnrho"][2] == pytest.approx(np.sqrt(np.mean(lr * lr)), rel=1e-13)
+    a2 = sum(inner[abi.VTX[k]] ** 2 for k in ("VTXBUF_AX", "VTXBUF_AY", "VTXBUF_AZ"))
+    assert d["aa"][2] == pytest.approx(np.sqrt(np.mean(a2)), rel=1e-13)
