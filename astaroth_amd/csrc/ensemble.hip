@@ -322,13 +322,15 @@ enqueue_diagnostics(Ensemble e, int par)
 AcResult
 enqueue_step(Ensemble e, int par0)
 {
+    // The batched solve reads periodic images instead of ghost cells, so the substeps need no
+    // fill; one fill after the last substep leaves every box's ghosts current.
     int par = par0;
     for (int s = 0; s < 3; ++s) {
-        AC_HIP_CHECK(enqueue_fill(e, par));
         AC_HIP_CHECK(launch_substep_batch(e->tile_x, e->tile_y, e->d_args[s][par], e->d_items,
                                           e->nitems, e->nblocks, e->stream));
         par = 1 - par;
     }
+    AC_HIP_CHECK(enqueue_fill(e, par));
     AC_HIP_CHECK(enqueue_diagnostics(e, par));
     return AC_SUCCESS;
 }

@@ -619,8 +619,11 @@ solve_zmarch2(const SubstepArgs a, const int zdepth, const int ntx, const int nt
 // WIN: every stencil read of one field in one ring plane goes through one LDS base register
 // (the window's first cell, pinned in a VGPR), so that all its reads pair into ds_read2 with
 // immediate offsets; otherwise the compiler forms a new base (one VALU add) for most pairs.
+// WRAP: the tile halos and the planes above and below the box are read at their periodic images
+// inside the interior, so no ghost cell is read and the ghosts need not be filled first (the
+// values read are the same as after a periodic fill).
 template <int TX, int TY, bool COMP = false, int NSLOT = 4, bool DEFER = false,
-          bool PREV_FIRST = false, int NQ = 7, bool WIN = false>
+          bool PREV_FIRST = false, int NQ = 7, bool WIN = false, bool WRAP = false>
 struct ZMarchLean {
     static_assert(NQ == 7 || (NQ == 8 && NSLOT == 4), "8-phase loop needs the 4-slot ring");
     static constexpr int NT  = TX * TY;
@@ -689,7 +692,12 @@ struct ZMarchLean {
 
     __device__ __forceinline__ void issue_loads(State& st, int p) const
     {
-        const long pl = (long)min(p, zmax) * a.mxy;
+        int q = min(p, zmax);
+        if (WRAP) { // periodic image of a ghost plane (planes run 0 .. zmax = nz + 5)
+            const int nz = zmax - 5;
+            q = q < NGHOST ? q + nz : q >= nz + NGHOST ? q - nz : q;
+        }
+        const long pl = (long)q * a.mxy;
         unsigned go[R];
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -1381,13 +1389,13 @@ strip_tile(const int b, const int ntx, const int nty, int& btx, int& bty, int& b
 
 // One z-march column: tile (btx, bty) of box `a`, output planes [zs, min(zs + zdepth, end.z)).
 template <int TX, int TY, bool COMP = false, int NSLOT = 4, bool DEFER = false,
-          bool PREV_FIRST = false, int NQ = 7, bool WIN = false>
+          bool PREV_FIRST = false, int NQ = 7, bool WIN = false, bool WRAP = false>
 __device__ __forceinline__ void
 zlean_column(const SubstepArgs& a,
-             typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN>::Shared& sh,
+             typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN, WRAP>::Shared& sh,
              const int btx, const int bty, const int zs, const int zdepth)
 {
-    typedef ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN> K;
+    typedef ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN, WRAP> K;
     K k(a, sh);
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = lean_tile_row<TX, TY>(tid);
@@ -1402,9 +1410,14 @@ zlean_column(const SubstepArgs& a,
     for (int r = 0; r < K::R; ++r) {
         const int e  = tid + r * K::NT;
         const int ex = e % K::LX, ey = e / K::LX;
-        const int xx = x0 - 3 + ex, yy = y0 - 3 + ey;
-        k.goff[r] = (unsigned)((e < K::LN && xx < a.mx && yy < a.my) ? xx + yy * a.mx : 0) *
-                    sizeof(AcReal);
+        int xx = x0 - 3 + ex, yy = y0 - 3 + ey;
+        const bool in_box = e < K::LN && xx < a.mx && yy < a.my;
+        if (WRAP) { // periodic images of the x and y ghost columns
+            const int nx = a.mx - 2 * NGHOST, ny = a.my - 2 * NGHOST;
+            xx = xx < NGHOST ? xx + nx : xx >= nx + NGHOST ? xx - nx : xx;
+            yy = yy < NGHOST ? yy + ny : yy >= ny + NGHOST ? yy - ny : yy;
+        }
+        k.goff[r] = (unsigned)(in_box ? xx + yy * a.mx : 0) * sizeof(AcReal);
     }
     k.cofs = (ty + 3) * K::LX + (tx + 3);
 
@@ -1487,11 +1500,13 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W))
 solve_zlean_batch(const SubstepArgs* __restrict__ boxes, const BatchItem* __restrict__ items,
                   const int nitems)
 {
-    __shared__ typename ZMarchLean<TX, TY, false, NSLOT, DEFER, PREV_FIRST, 7, WIN>::Shared sh;
+    // periodic images read directly: the ensemble step fills no ghosts between its substeps
+    __shared__
+        typename ZMarchLean<TX, TY, false, NSLOT, DEFER, PREV_FIRST, 7, WIN, true>::Shared sh;
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const BatchItem w = items[it];
-        zlean_column<TX, TY, false, NSLOT, DEFER, PREV_FIRST, 7, WIN>(boxes[w.box], sh, w.btx,
-                                                                      w.bty, w.zs, w.zdepth);
+        zlean_column<TX, TY, false, NSLOT, DEFER, PREV_FIRST, 7, WIN, true>(
+            boxes[w.box], sh, w.btx, w.bty, w.zs, w.zdepth);
     }
 }
 

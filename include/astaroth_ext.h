@@ -140,9 +140,11 @@ AcResult acDeviceHaloExchangeSelf(const Device device, const Stream stream);
 /* ---- Batched ensembles of independent periodic boxes ----
  * No counterpart in the reference (one box per Device there); for callers that advance many
  * small boxes of uneven sizes, e.g. parameter scans. Each box has its own AcMeshInfo (grid size,
- * spacing, viscosity, ...) and the reference layout. One RK3 step is 3 x {one periodic-fill launch
- * over all boxes, one persistent solve launch over a (box, tile column, z range) work list} plus
- * per-box FP64 diagnostics, on the ensemble's own stream. All boxes share dt.
+ * spacing, viscosity, ...) and the reference layout. One RK3 step is 3 persistent solve launches
+ * over a (box, tile column, z range) work list, which read the periodic images of the ghost
+ * cells directly (the same values a periodic fill would give), then one periodic-fill launch over
+ * all boxes (so the ghosts are current after the step) and per-box FP64 diagnostics, on the
+ * ensemble's own stream. All boxes share dt.
  * Box fields are set with acEnsembleRandomizeUniform / acEnsembleSetVertexBuffer (device) or
  * acEnsembleLoadMesh (host, synchronous); acEnsemblePeriodicBoundconds fills the ghosts. */
 typedef struct ensemble_s* Ensemble;

@@ -149,3 +149,43 @@ def test_ensemble_cfl_timestep(lib):
         uu_dt = rp("AC_cdt") * dsmin / (d[b, 0] + math.sqrt(rp("AC_cs2_sound")))
         visc_dt = rp("AC_cdtv") * dsmin ** 2 / max(max(rp("AC_nu_visc"), rp("AC_eta")), rp("AC_gamma"))
         assert dts[b] == pytest.approx(min(uu_dt, visc_dt), rel=1e-12)
+
+
+@pytest.mark.parametrize("config", [(16, 16, 0), (32, 4, 14), (16, 8, 0)])
+def test_batched_step_reads_no_ghost_cell(lib, config):
+    """The batched solve reads periodic images, not ghost cells: a state whose ghosts hold NaN
+    steps bitwise like the same state with filled ghosts, and after the step every ghost cell
+    holds its periodic image (the step's final fill)."""
+    infos = box_infos(lib)
+    rng = np.random.default_rng(12)
+    meshes = []
+    for info, (nx, ny, nz) in zip(infos, BOXES):
+        interior = [rng.uniform(-1e-3, 1e-3, (nz, ny, nx)) for _ in range(8)]
+        meshes.append([np.pad(f, 3, mode="wrap") for f in interior])
+    outs = []
+    for poison in (False, True):
+        with lib.ensemble_create(infos) as ens:
+            ens.set_kernel_config(*config)
+            for b, info in enumerate(infos):
+                fields = [f.copy() for f in meshes[b]]
+                if poison:
+                    for f in fields:
+                        f[:3] = np.nan
+                        f[-3:] = np.nan
+                        f[:, :3] = np.nan
+                        f[:, -3:] = np.nan
+                        f[:, :, :3] = np.nan
+                        f[:, :, -3:] = np.nan
+                ens.load_mesh(b, HostMesh(lib, info, fields))
+            ens.integrate_step(1e-3, mode="batched")
+            res = []
+            for b, info in enumerate(infos):
+                m = HostMesh(lib, info)
+                ens.store_mesh(b, m)
+                res.append(m)
+        outs.append(res)
+    for b in range(len(infos)):
+        for q in range(8):
+            a, p = outs[0][b].fields[q], outs[1][b].fields[q]
+            assert np.array_equal(a, p), (BOXES[b], q)
+            assert np.array_equal(a, np.pad(a[3:-3, 3:-3, 3:-3], 3, mode="wrap")), (BOXES[b], q)
