@@ -74,6 +74,9 @@ struct SubstepArgs {
     AcReal* __restrict__ cout[NF];
     int mx, my, mz, mxy;
     int3 start, end;
+    // 1: the tile halos and the planes beyond the interior are read at their periodic images
+    // inside the interior, so no ghost cell is read (configurations of kernel_config_reads_images)
+    int wrap;
     PhysParams phys;
     RK3Coeffs rk;
 };
@@ -91,6 +94,8 @@ RK3Coeffs make_rk3_coeffs(int step, AcReal dt);
 // one of the compiled configurations.
 hipError_t launch_substep(const AcKernelConfig& config, const SubstepArgs& args, hipStream_t stream);
 bool kernel_config_supported(const AcKernelConfig& config);
+// True when `config` has a build that reads periodic images (SubstepArgs::wrap = 1).
+bool kernel_config_reads_images(const AcKernelConfig& config);
 // Batched solve (persistent lean z-march over a work list): `boxes` and `items` are device
 // arrays; tile shapes 32x4, 16x8, 16x16.
 bool batch_tile_supported(int tile_x, int tile_y);

@@ -102,6 +102,7 @@ make_args(const Device d, const AcReal* const* in, AcReal* const* out, int step,
     a.mxy   = a.mx * a.my;
     a.start = start;
     a.end   = end;
+    a.wrap  = 0;
     a.phys  = make_phys_params(uniforms_of(d));
     a.rk    = make_rk3_coeffs(step, dt);
     return a;
@@ -1000,6 +1001,17 @@ acDeviceRunMPITest(void)
     return AC_FAILURE;
 }
 
+// The step's substeps read the periodic images of the ghost cells instead of filling them
+// first: the configuration has such a build, compensated integration is off and the interior
+// spans at least one ghost width on each axis.
+static bool
+step_reads_images(const Device d)
+{
+    if (d->comp[0][0] != nullptr || nx_of(d) < NGHOST || ny_of(d) < NGHOST || nz_of(d) < NGHOST)
+        return false;
+    return kernel_config_reads_images(d->kernel_config);
+}
+
 AcResult
 acDeviceIntegrateStep(const Device device, const Stream stream, const AcReal dt)
 {
@@ -1007,6 +1019,22 @@ acDeviceIntegrateStep(const Device device, const Stream stream, const AcReal dt)
     const int3 m    = mk3(nx_of(device) + 6, ny_of(device) + 6, nz_of(device) + 6);
     const int3 nmin = mk3(NGHOST, NGHOST, NGHOST);
     const int3 nmax = mk3(nx_of(device) + NGHOST, ny_of(device) + NGHOST, nz_of(device) + NGHOST);
+    if (step_reads_images(device)) {
+        // The build reads the periodic images of the ghost cells itself: the substeps need no
+        // fill between them and compute what they compute after one.
+        if (!stream_ok(device, stream) ||
+            acDeviceLoadScalarUniform(device, stream, AC_dt, dt) != AC_SUCCESS)
+            return AC_FAILURE;
+        AC_HIP_CHECK(hipSetDevice(device->id));
+        hipStream_t hs = device->streams[stream];
+        for (int s = 0; s < 3; ++s) {
+            SubstepArgs a = make_args(device, device->vba.in, device->vba.out, s, nmin, nmax, dt);
+            a.wrap        = 1;
+            AC_HIP_CHECK(launch_substep(device->kernel_config, a, hs));
+            acDeviceSwapBuffers(device);
+        }
+        return AC_SUCCESS;
+    }
     for (int s = 0; s < 3; ++s) {
         if (acDevicePeriodicBoundconds(device, stream, zero, m) != AC_SUCCESS ||
             acDeviceIntegrateSubstep(device, stream, s, nmin, nmax, dt) != AC_SUCCESS)
@@ -1017,11 +1045,20 @@ acDeviceIntegrateStep(const Device device, const Stream stream, const AcReal dt)
 }
 
 // The step of acNodeIntegrateGBC (node.cc:808-880) on one Device: 3 x {general boundary fill by
-// the face types of `config`, substep over the interior, swap}.
+// the face types of `config`, substep over the interior, swap}. With every face periodic it is
+// acDeviceIntegrateStep.
 AcResult
 acDeviceIntegrateStepGBC(const Device device, const Stream stream, const AcMeshInfo config,
                          const AcReal dt)
 {
+    int bot[3], top[3];
+    if (!general_bc_faces(device, config, mk3(-1, -1, -1), bot, top))
+        return AC_FAILURE;
+    bool periodic = true;
+    for (int d = 0; d < 3; ++d)
+        periodic = periodic && bot[d] == AC_BOUNDCOND_PERIODIC && top[d] == AC_BOUNDCOND_PERIODIC;
+    if (periodic)
+        return acDeviceIntegrateStep(device, stream, dt);
     const int3 zero  = mk3(0, 0, 0);
     const int3 m     = mk3(nx_of(device) + 6, ny_of(device) + 6, nz_of(device) + 6);
     const int3 nmin  = mk3(NGHOST, NGHOST, NGHOST);
@@ -1059,12 +1096,15 @@ acDeviceIntegrateStepGraph(const Device device, const Stream stream, const AcRea
             bufs[0][q] = device->vba.in[q];
             bufs[1][q] = device->vba.out[q];
         }
+        const bool images = step_reads_images(device);
         AC_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (int st = 0; st < 3; ++st) {
             AcReal* const* cin  = bufs[st & 1];
             AcReal* const* cout = bufs[(st + 1) & 1];
-            AC_HIP_CHECK(launch_periodic_fill(cin, NF, nx, ny, nz, zero, m, s));
-            const SubstepArgs a = make_args(device, cin, cout, st, nmin, nmax, dt, st & 1);
+            if (!images)
+                AC_HIP_CHECK(launch_periodic_fill(cin, NF, nx, ny, nz, zero, m, s));
+            SubstepArgs a = make_args(device, cin, cout, st, nmin, nmax, dt, st & 1);
+            a.wrap        = images;
             AC_HIP_CHECK(launch_substep(device->kernel_config, a, s));
         }
         hipGraph_t graph;

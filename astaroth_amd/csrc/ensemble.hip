@@ -273,6 +273,7 @@ build_args(Ensemble e, AcReal dt)
                 x.mxy   = x.mx * x.my;
                 x.start = make_int3(NGHOST, NGHOST, NGHOST);
                 x.end   = make_int3(n.x + NGHOST, n.y + NGHOST, n.z + NGHOST);
+                x.wrap  = 1;
                 x.phys  = make_phys_params(e->info[b]);
                 x.rk    = make_rk3_coeffs(s, dt);
             }
@@ -643,6 +644,7 @@ acEnsembleIntegrateStepLooped(Ensemble e, const AcReal dt)
             a.mxy   = a.mx * a.my;
             a.start = make_int3(NGHOST, NGHOST, NGHOST);
             a.end   = make_int3(n.x + NGHOST, n.y + NGHOST, n.z + NGHOST);
+            a.wrap  = 0;
             a.phys  = make_phys_params(e->info[b]);
             a.rk    = make_rk3_coeffs(s, dt);
             AC_HIP_CHECK(launch_substep(default_kernel_config(n.x, n.y, n.z), a, e->stream));

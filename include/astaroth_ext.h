@@ -27,14 +27,18 @@ AcResult acDeviceReduceVecScalDouble(const Device device, const Stream stream,
                                      const VertexBufferHandle vtxbuf3, double* result);
 
 /* ---- Whole RK3 step ----
- * Three times {periodic fill of `in` over the whole m^3 box, substep over the interior, swap},
- * the caller sequence of the reference's single-GPU node loop (src/core/node.cc:729-806).
- * The state ends in `in` without a final fill, as in the reference. */
+ * The interior result of three times {periodic fill of `in` over the whole m^3 box, substep over
+ * the interior, swap}, the caller sequence of the reference's single-GPU node loop
+ * (src/core/node.cc:729-806). In the FP64 library with its default configuration (variant 8,
+ * 16 x 16 tiles), compensated integration off and n >= 3 on every axis, the substeps read the
+ * periodic images of the ghost cells directly and no fill is made; otherwise `in` is filled
+ * before each substep. The state ends in `in` without a final fill, as in the reference: its
+ * ghost cells are to be filled before they are read. */
 AcResult acDeviceIntegrateStep(const Device device, const Stream stream, const AcReal dt);
 
 /* The step with general boundaries (acNodeIntegrateGBC, node.cc:808-880, on one Device):
  * 3 x {acDeviceGeneralBoundconds over the whole box with the face types of `config`, substep,
- * swap}. */
+ * swap}; acDeviceIntegrateStep when every face is periodic. */
 AcResult acDeviceIntegrateStepGBC(const Device device, const Stream stream,
                                   const AcMeshInfo config, const AcReal dt);
 

@@ -384,6 +384,59 @@ def test_beltrami_decay(lib):
     assert np.max(np.abs(gpu[5][sl] - an * np.cos(k * zc[sl]))) < tol
 
 
+@pytest.mark.parametrize("zd", [0, 7])
+def test_step_reads_periodic_images(lib, zd):
+    """acDeviceIntegrateStep in the FP64 library with its default configuration makes no fill:
+    its substeps read the periodic images of the ghost cells (the FP32 default and variant 2 fill
+    before each substep). In each case the step's interior equals, bit for bit, that of the fill
+    sequence made by hand (3 x {acDevicePeriodicBoundconds, acDeviceIntegrateSubstep, swap}),
+    with every ghost cell of both buffer sets NaN (a ghost read would carry a NaN into the
+    interior); the same for the recorded step and for the general-boundary step with periodic
+    faces."""
+    nx, ny, nz = 21, 18, 15
+    dt = 1e-3
+    info = lib.config(n=nx, ny=ny, nz=nz)
+    fields = mhd_fields(nx, ny, nz, np.float64, config_id=1)
+    nan_ghosts = []
+    for f in fields:
+        g = np.full_like(f, np.nan)
+        g[3:-3, 3:-3, 3:-3] = f[3:-3, 3:-3, 3:-3]
+        nan_ghosts.append(g)
+
+    def loaded(config):
+        dev = make_device(lib, info)
+        dev.set_kernel_config(*config)
+        mesh = host_mesh_from(lib, info, nan_ghosts)
+        dev.load_mesh(mesh)  # both buffer sets: in, then out through a swap
+        dev.swap_buffers()
+        dev.load_mesh(mesh)
+        dev.swap_buffers()
+        return dev, mesh
+
+    lean = (8 if lib.double else 19, 16, 16, zd)
+    for config, modes in ((lean, ("eager", "graph", "gbc")), ((2, 32, 4, zd), ("eager",))):
+        dev, mesh = loaded(config)
+        with dev:
+            for s in range(3):
+                dev.periodic_boundconds()
+                assert dev.integrate_substep(s, dt=dt) == abi.AC_SUCCESS
+                dev.swap_buffers()
+            ref = mesh.copy()
+            dev.store_mesh(ref)
+        assert all(np.isfinite(ref.interior(q)).all() for q in range(8))
+        for mode in modes:
+            dev, mesh = loaded(config)
+            with dev:
+                if mode == "gbc":
+                    dev.integrate_step_gbc(info, lib.real(dt))
+                else:
+                    dev.integrate_step(dt, graph=(mode == "graph"))
+                out = mesh.copy()
+                dev.store_mesh(out)
+            for q in range(8):
+                assert np.array_equal(out.interior(q), ref.interior(q)), (mode, config, FIELD_NAMES[q])
+
+
 def test_graph_step_matches_eager(lib):
     """acDeviceIntegrateStepGraph replays give the same state as eager steps, replay after
     replay, with eager work and synchronisation in between."""
