@@ -622,14 +622,25 @@ solve_zmarch2(const SubstepArgs a, const int zdepth, const int ntx, const int nt
 // WRAP: the tile halos and the planes above and below the box are read at their periodic images
 // inside the interior, so no ghost cell is read and the ghosts need not be filled first (the
 // values read are the same as after a periodic fill).
+// DMA (5-slot ring only): plane p + 1 is loaded straight into its ring slot with direct-to-LDS
+// loads (global_load_lds_dword: lane l of a wave writes the 4 bytes it loads to the wave's LDS
+// base + 4 l), issued at the start of step p into the slot that plane p - 4 left; no staging
+// registers and no LDS writes. Each wave waits for its own loads (vmcnt 0) before the step's
+// barrier, so after it the whole plane is in LDS. In FP64 a round of 64 doubles is two such
+// loads, lane l fetching the low or high half (l & 1) of double l / 2 of its half-round.
 template <int TX, int TY, bool COMP = false, int NSLOT = 4, bool DEFER = false,
-          bool PREV_FIRST = false, int NQ = 7, bool WIN = false, bool WRAP = false>
+          bool PREV_FIRST = false, int NQ = 7, bool WIN = false, bool WRAP = false,
+          bool DMA = false>
 struct ZMarchLean {
     static_assert(NQ == 7 || (NQ == 8 && NSLOT == 4), "8-phase loop needs the 4-slot ring");
+    static_assert(!DMA || NSLOT == 5, "direct-to-LDS staging needs the 5-slot ring");
     static constexpr int NT  = TX * TY;
+    static_assert(NT % 64 == 0, "whole waves");
     static constexpr int LX  = TX + 6, LY = TY + 6, LN = LX * LY;
     static constexpr int R   = (LN + NT - 1) / NT;
     static constexpr int LNP = R * NT;
+    // loads per field and round: the direct-to-LDS load moves 4 bytes per lane
+    static constexpr int NH = DMA ? (int)(sizeof(AcReal) / 4) : 1;
 
     struct Shared {
         AcReal ring[NSLOT][NF][LNP];
@@ -638,7 +649,7 @@ struct ZMarchLean {
     struct State {
         AcReal qv[NF][3]; // planes z-3..z-1 of every field
         AcReal x[6][NQ];  // cross-derivative sums, slot (z - pbeg) mod NQ
-        AcReal pv[NF][R]; // staging registers of the next plane
+        AcReal pv[DMA ? 1 : NF][DMA ? 1 : R]; // staging registers of the next plane
         AcReal res[NF];   // updated values of plane res_z, stored during the next step
         int res_z;        // -1: nothing pending
     };
@@ -661,9 +672,10 @@ struct ZMarchLean {
     const SubstepArgs& a;
     Shared& sh;
     int tid, cofs, ze, zmax;
+    int wave; // wave index within the block (uniform)
     unsigned own; // byte offset of the thread's column within a plane
     bool active;
-    unsigned goff[R];
+    unsigned goff[R][NH]; // byte offsets within a plane of what this lane loads per round
 
     __device__ __forceinline__ ZMarchLean(const SubstepArgs& args, Shared& shared) : a(args), sh(shared) {}
 
@@ -683,6 +695,7 @@ struct ZMarchLean {
 
     __device__ __forceinline__ void stage(const State& st, int slot) const
     {
+        static_assert(!DMA, "direct-to-LDS loads need no staging");
 #pragma unroll
         for (int f = 0; f < NF; ++f)
 #pragma unroll
@@ -698,18 +711,40 @@ struct ZMarchLean {
             q = q < NGHOST ? q + nz : q >= nz + NGHOST ? q - nz : q;
         }
         const long pl = (long)q * a.mxy;
-        unsigned go[R];
+        unsigned go[R][NH];
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            go[r] = pin(goff[r]);
 #pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            const AcReal* src = a.in[f] + pl;
+            for (int h = 0; h < NH; ++h)
+                go[r][h] = pin(goff[r][h]);
+        if constexpr (DMA) {
+            const int slot = ring_slot(p);
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                st.pv[f][r] = ld(src, go[r]);
+            for (int f = 0; f < NF; ++f) {
+                const char* src = (const char*)(a.in[f] + pl);
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int h = 0; h < NH; ++h) {
+                        AcReal* dst = &sh.ring[slot][f][r * NT + wave * 64 + h * (64 / NH)];
+                        __builtin_amdgcn_global_load_lds(
+                            src + go[r][h], (__attribute__((address_space(3))) void*)dst, 4, 0, 0);
+                    }
+            }
+        }
+        else {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                const AcReal* src = a.in[f] + pl;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    st.pv[f][r] = ld(src, go[r][0]);
+            }
         }
     }
+
+    // Wait for this wave's outstanding memory operations (vmcnt 0; expcnt and lgkmcnt free).
+    static __device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
     __device__ __forceinline__ void push_queue(State& st, const AcReal* ro) const
     {
@@ -767,7 +802,7 @@ struct ZMarchLean {
         // ring slot of plane p (and below, of planes z = p - 3, z + 1, z + 2)
         const int slot = NQ == 8 ? (PH & 3) : ring_slot(p);
 
-        if (NSLOT == 4) {
+        if constexpr (NSLOT == 4) {
             __syncthreads(); // every thread is done with ring slot `slot` (plane p - 4)
             stage(st, slot);
             __syncthreads();
@@ -775,8 +810,11 @@ struct ZMarchLean {
         else {
             // plane p (staged during step p - 1) is visible, and every thread is done with
             // plane p - 4, whose slot now receives plane p + 1
+            if constexpr (DMA)
+                wait_vm(); // this wave's part of plane p has landed
             __syncthreads();
-            stage(st, ring_slot(p + 1));
+            if constexpr (!DMA)
+                stage(st, ring_slot(p + 1));
         }
 
         if (DEFER)
@@ -784,8 +822,11 @@ struct ZMarchLean {
         // With PREV_FIRST the previous stage's values are loaded before the next plane: memory
         // operations complete in issue order, so waiting for them at the update then does not
         // also wait for the next plane's loads (measured faster in FP64, slower in FP32).
+        // next plane to load: into registers, p + 1 (4 slots) or p + 2 (5 slots, staged at the
+        // next step); straight into its ring slot, p + 1
+        constexpr int AHEAD = (NSLOT == 4 || DMA) ? 1 : 2;
         if (!PREV_FIRST)
-            issue_loads(st, NSLOT == 4 ? p + 1 : p + 2);
+            issue_loads(st, p + AHEAD);
         const int z             = p - 3;
         const unsigned long oz  = (unsigned long)min(z, ze - 1) * a.mxy;
         const unsigned ow       = pin(own);
@@ -812,7 +853,7 @@ struct ZMarchLean {
             }
         }
         if (PREV_FIRST)
-            issue_loads(st, NSLOT == 4 ? p + 1 : p + 2);
+            issue_loads(st, p + AHEAD);
 
         // ---- ingest plane p: x-z and y-z differences into the pending sums ----
         const AcReal* __restrict__ rs = &sh.ring[slot][0][0];
@@ -1389,15 +1430,17 @@ strip_tile(const int b, const int ntx, const int nty, int& btx, int& bty, int& b
 
 // One z-march column: tile (btx, bty) of box `a`, output planes [zs, min(zs + zdepth, end.z)).
 template <int TX, int TY, bool COMP = false, int NSLOT = 4, bool DEFER = false,
-          bool PREV_FIRST = false, int NQ = 7, bool WIN = false, bool WRAP = false>
+          bool PREV_FIRST = false, int NQ = 7, bool WIN = false, bool WRAP = false,
+          bool DMA = false>
 __device__ __forceinline__ void
 zlean_column(const SubstepArgs& a,
-             typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN, WRAP>::Shared& sh,
+             typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN, WRAP, DMA>::Shared& sh,
              const int btx, const int bty, const int zs, const int zdepth)
 {
-    typedef ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN, WRAP> K;
+    typedef ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN, WRAP, DMA> K;
     K k(a, sh);
     const int tid = threadIdx.x;
+    k.wave        = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int tx = tid % TX, ty = lean_tile_row<TX, TY>(tid);
     const int x0 = a.start.x + btx * TX, y0 = a.start.y + bty * TY;
     k.tid        = tid;
@@ -1407,18 +1450,24 @@ zlean_column(const SubstepArgs& a,
     k.active     = gx < a.end.x && gy < a.end.y;
     k.own        = (unsigned)(k.active ? gx + gy * a.mx : x0 + y0 * a.mx) * sizeof(AcReal);
 #pragma unroll
-    for (int r = 0; r < K::R; ++r) {
-        const int e  = tid + r * K::NT;
-        const int ex = e % K::LX, ey = e / K::LX;
-        int xx = x0 - 3 + ex, yy = y0 - 3 + ey;
-        const bool in_box = e < K::LN && xx < a.mx && yy < a.my;
-        if (WRAP) { // periodic images of the x and y ghost columns
-            const int nx = a.mx - 2 * NGHOST, ny = a.my - 2 * NGHOST;
-            xx = xx < NGHOST ? xx + nx : xx >= nx + NGHOST ? xx - nx : xx;
-            yy = yy < NGHOST ? yy + ny : yy >= ny + NGHOST ? yy - ny : yy;
+    for (int r = 0; r < K::R; ++r)
+#pragma unroll
+        for (int h = 0; h < K::NH; ++h) {
+            // the tile cell (haloed LX x LY tile, row-major) this lane loads in round r: its own,
+            // or with two loads per double the one whose half it fetches
+            const int e  = K::NH == 1 ? tid + r * K::NT
+                                      : r * K::NT + (tid & ~63) + h * 32 + ((tid & 63) >> 1);
+            const int ex = e % K::LX, ey = e / K::LX;
+            int xx = x0 - 3 + ex, yy = y0 - 3 + ey;
+            const bool in_box = e < K::LN && xx < a.mx && yy < a.my;
+            if (WRAP) { // periodic images of the x and y ghost columns
+                const int nx = a.mx - 2 * NGHOST, ny = a.my - 2 * NGHOST;
+                xx = xx < NGHOST ? xx + nx : xx >= nx + NGHOST ? xx - nx : xx;
+                yy = yy < NGHOST ? yy + ny : yy >= ny + NGHOST ? yy - ny : yy;
+            }
+            k.goff[r][h] = (unsigned)(in_box ? xx + yy * a.mx : 0) * sizeof(AcReal) +
+                           (K::NH == 1 ? 0u : (unsigned)(tid & 1) * 4u);
         }
-        k.goff[r] = (unsigned)(in_box ? xx + yy * a.mx : 0) * sizeof(AcReal);
-    }
     k.cofs = (ty + 3) * K::LX + (tx + 3);
 
     typename K::State st;
@@ -1437,11 +1486,17 @@ zlean_column(const SubstepArgs& a,
             st.x[n][j] = 0;
 
     const int pbeg = zs - 3;
-    k.issue_loads(st, pbeg);
-    if (NSLOT == 5) {
+    if constexpr (DMA) {
         __syncthreads(); // the previous column of this block is done with the ring
-        k.stage(st, K::ring_slot(pbeg));
-        k.issue_loads(st, pbeg + 1);
+        k.issue_loads(st, pbeg);
+    }
+    else {
+        k.issue_loads(st, pbeg);
+        if (NSLOT == 5) {
+            __syncthreads(); // the previous column of this block is done with the ring
+            k.stage(st, K::ring_slot(pbeg));
+            k.issue_loads(st, pbeg + 1);
+        }
     }
     k.template step<0, false>(st, pbeg);
     k.template step<1, false>(st, pbeg + 1);
@@ -1474,17 +1529,21 @@ zlean_column(const SubstepArgs& a,
     }
     if (DEFER)
         k.flush(st); // the last output plane
+    if constexpr (DMA)
+        K::wait_vm(); // no load into the ring is left in flight when the block moves on or ends
 }
 
 template <int TX, int TY, int W, bool COMP = false, int NSLOT = 4, bool DEFER = false,
-          bool PREV_FIRST = false, int NQ = 7, bool WIN = false, bool WRAP = false>
+          bool PREV_FIRST = false, int NQ = 7, bool WIN = false, bool WRAP = false,
+          bool DMA = false>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W)))
 solve_zlean(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
 {
-    __shared__ typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN, WRAP>::Shared sh;
+    __shared__
+        typename ZMarchLean<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN, WRAP, DMA>::Shared sh;
     int btx, bty, btz;
     strip_tile(xcd_remap((int)blockIdx.x, (int)gridDim.x), ntx, nty, btx, bty, btz);
-    zlean_column<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN, WRAP>(
+    zlean_column<TX, TY, COMP, NSLOT, DEFER, PREV_FIRST, NQ, WIN, WRAP, DMA>(
         a, sh, btx, bty, a.start.z + btz * zdepth, zdepth);
 }
 
@@ -1563,6 +1622,7 @@ constexpr ConfigEntry kConfigs[] = {
     {17, 16, 16},
     {19, 16, 16},
     {22, 16, 16},
+    {23, 16, 16},
 #endif
     {8, 16, 16},
 };
@@ -1784,6 +1844,11 @@ launch_substep(const AcKernelConfig& c, const SubstepArgs& a, hipStream_t stream
     if (c.variant == 22 && c.tile_x == 16 && c.tile_y == 16)
         return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2, false, 4, true, false, 7, true>,
                                           a, c.z_depth, stream);
+    // direct-to-LDS plane loads into a 5-slot ring, pinned LDS windows
+    if (c.variant == 23 && c.tile_x == 16 && c.tile_y == 16)
+        return launch_zmarch_grid<16, 16>(
+            solve_zlean<16, 16, 2, false, 5, true, true, 7, true, false, true>, a, c.z_depth,
+            stream);
 #endif
     if (c.variant == 8 && c.tile_x == 16 && c.tile_y == 16)
         return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 1, false, 4, true, true>, a,

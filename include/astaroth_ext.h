@@ -61,7 +61,9 @@ AcResult acDeviceIntegrateStepGraph(const Device device, const Stream stream, co
  * variant 13: variant 12 with an 8-plane loop, whose LDS ring slots are compile-time (FP32).
  * variant 16: variant 12 with the 5-slot ring (one barrier per plane).
  * variants 17, 19, 22: variants 16, 12 and 10 reading each field's LDS stencil window through
- *            one pinned base register (FP32; variant 19 is the FP32 default). */
+ *            one pinned base register (FP32; variant 19 is the FP32 default).
+ * variant 23: variant 17 with each plane loaded straight into its ring slot by direct-to-LDS
+ *            loads (no staging registers or LDS writes; FP32). */
 typedef struct {
     int variant;
     int tile_x;
