@@ -59,6 +59,17 @@ class Int3(ctypes.Structure):
         return iter((self.x, self.y, self.z))
 
 
+class GridDims(ctypes.Structure):
+    """GridDims of include/astaroth.h: m (with ghosts), n (interior)."""
+    _fields_ = [("m", Int3), ("n", Int3)]
+
+
+class DeviceConfiguration(ctypes.Structure):
+    """DeviceConfiguration of include/astaroth.h (acNodeQueryDeviceConfiguration)."""
+    _fields_ = [("num_devices", ctypes.c_int), ("devices", ctypes.POINTER(ctypes.c_void_p)),
+                ("grid", GridDims), ("subgrid", GridDims)]
+
+
 def int3(x, y, z) -> Int3:
     return Int3(int(x), int(y), int(z))
 

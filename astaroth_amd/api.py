@@ -114,6 +114,40 @@ class Library:
             "acDeviceReduceVecDouble": (i, [vp, i, i, i, i, i, ctypes.POINTER(ctypes.c_double)]),
             "acDeviceReduceVecScalDouble": (i, [vp, i, i, i, i, i, i, ctypes.POINTER(ctypes.c_double)]),
             "acDeviceRunMPITest": (i, []),
+            "acNodeCreate": (i, [i, MI, ctypes.POINTER(vp)]),
+            "acNodeDestroy": (i, [vp]),
+            "acNodePrintInfo": (i, [vp]),
+            "acNodeQueryDeviceConfiguration": (i, [vp, ctypes.POINTER(abi.DeviceConfiguration)]),
+            "acNodeAutoOptimize": (i, [vp]),
+            "acNodeSynchronizeStream": (i, [vp, i]),
+            "acNodeSynchronizeVertexBuffer": (i, [vp, i, i]),
+            "acNodeSynchronizeMesh": (i, [vp, i]),
+            "acNodeSwapBuffers": (i, [vp]),
+            "acNodeLoadConstant": (i, [vp, i, i, R]),
+            "acNodeLoadVertexBufferWithOffset": (i, [vp, i, M, i, I3, I3, i]),
+            "acNodeLoadMeshWithOffset": (i, [vp, i, M, I3, I3, i]),
+            "acNodeLoadVertexBuffer": (i, [vp, i, M, i]),
+            "acNodeLoadMesh": (i, [vp, i, M]),
+            "acNodeSetVertexBuffer": (i, [vp, i, i, R]),
+            "acNodeStoreVertexBufferWithOffset": (i, [vp, i, i, I3, I3, i, ctypes.POINTER(M)]),
+            "acNodeStoreMeshWithOffset": (i, [vp, i, I3, I3, i, ctypes.POINTER(M)]),
+            "acNodeStoreVertexBuffer": (i, [vp, i, i, ctypes.POINTER(M)]),
+            "acNodeStoreMesh": (i, [vp, i, ctypes.POINTER(M)]),
+            "acNodeIntegrateSubstep": (i, [vp, i, i, I3, I3, R]),
+            "acNodeIntegrate": (i, [vp, R]),
+            "acNodeIntegrateGBC": (i, [vp, MI, R]),
+            "acNodePeriodicBoundcondStep": (i, [vp, i, i]),
+            "acNodePeriodicBoundconds": (i, [vp, i]),
+            "acNodeGeneralBoundcondStep": (i, [vp, i, i, MI]),
+            "acNodeGeneralBoundconds": (i, [vp, i, MI]),
+            "acNodeReduceScal": (i, [vp, i, i, i, ctypes.POINTER(R)]),
+            "acNodeReduceVec": (i, [vp, i, i, i, i, i, ctypes.POINTER(R)]),
+            "acNodeReduceVecScal": (i, [vp, i, i, i, i, i, i, ctypes.POINTER(R)]),
+            "acGetNode": (vp, []),
+            "acInit": (i, [MI]),
+            "acQuit": (i, []),
+            "acLoad": (i, [M]),
+            "acReduceScal": (R, [i, i]),
             "acDeviceIntegrateStep": (i, [vp, i, R]),
             "acDeviceIntegrateStepGraph": (i, [vp, i, R]),
             "acKernelConfigCount": (i, []),
@@ -233,6 +267,9 @@ class Library:
 
     def device_create(self, device_id: int, info, autotune: bool | None = None) -> "Device":
         return Device(self, device_id, info, autotune)
+
+    def node_create(self, info, node_id: int = 0, autotune: bool | None = None) -> "Node":
+        return Node(self, info, node_id, autotune)
 
     def ensemble_create(self, infos, device_id: int = 0) -> "Ensemble":
         return Ensemble(self, infos, device_id)
@@ -617,3 +654,91 @@ def _rt(r):
 
 def _vh(h):
     return VTX[h] if isinstance(h, str) else int(h)
+
+
+class Node:
+    """The Node interface (acNode*) of include/astaroth.h: one Device on GPU 0 for the whole grid."""
+
+    def __init__(self, lib: Library, info, node_id: int = 0, autotune=None):
+        self.lib = lib
+        self.info = info
+        self.dims = dims_of(info)
+        self.handle = ctypes.c_void_p()
+        old = os.environ.get("AC_AUTOTUNE")
+        if autotune is not None:
+            os.environ["AC_AUTOTUNE"] = "1" if autotune else "0"
+        try:
+            res = lib.dll.acNodeCreate(int(node_id), info, ctypes.byref(self.handle))
+        finally:
+            if autotune is not None:
+                if old is None:
+                    del os.environ["AC_AUTOTUNE"]
+                else:
+                    os.environ["AC_AUTOTUNE"] = old
+        if res != abi.AC_SUCCESS:
+            raise AstarothError("acNodeCreate failed")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+    def _check(self, res, what):
+        if res != abi.AC_SUCCESS:
+            raise AstarothError("%s returned AC_FAILURE" % what)
+        return res
+
+    def destroy(self):
+        if self.handle:
+            self.lib.dll.acNodeDestroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def synchronize(self, stream=abi.STREAM_ALL):
+        return self._check(self.lib.dll.acNodeSynchronizeStream(self.handle, stream), "acNodeSynchronizeStream")
+
+    def query_device_configuration(self) -> abi.DeviceConfiguration:
+        c = abi.DeviceConfiguration()
+        self._check(self.lib.dll.acNodeQueryDeviceConfiguration(self.handle, ctypes.byref(c)),
+                    "acNodeQueryDeviceConfiguration")
+        return c
+
+    def load_mesh(self, mesh: HostMesh, stream=abi.STREAM_DEFAULT):
+        self._check(self.lib.dll.acNodeLoadMesh(self.handle, stream, mesh.struct), "acNodeLoadMesh")
+        self.synchronize(stream)
+
+    def store_mesh(self, mesh: HostMesh, stream=abi.STREAM_DEFAULT):
+        self._check(self.lib.dll.acNodeStoreMesh(self.handle, stream, ctypes.byref(mesh.struct)), "acNodeStoreMesh")
+        self.synchronize(stream)
+        return mesh
+
+    def integrate(self, dt):
+        return self._check(self.lib.dll.acNodeIntegrate(self.handle, self.lib.real(dt)), "acNodeIntegrate")
+
+    def integrate_gbc(self, config, dt):
+        return self._check(self.lib.dll.acNodeIntegrateGBC(self.handle, config, self.lib.real(dt)),
+                           "acNodeIntegrateGBC")
+
+    def integrate_substep(self, step, start, end, dt, stream=abi.STREAM_DEFAULT):
+        return self._check(self.lib.dll.acNodeIntegrateSubstep(self.handle, stream, step, int3(*start),
+                                                               int3(*end), self.lib.real(dt)),
+                           "acNodeIntegrateSubstep")
+
+    def swap_buffers(self):
+        return self._check(self.lib.dll.acNodeSwapBuffers(self.handle), "acNodeSwapBuffers")
+
+    def periodic_boundconds(self, stream=abi.STREAM_DEFAULT):
+        self._check(self.lib.dll.acNodePeriodicBoundconds(self.handle, stream), "acNodePeriodicBoundconds")
+        self.synchronize(stream)
+
+    def reduce_scal(self, rtype, handle, stream=abi.STREAM_DEFAULT):
+        out = self.lib.real()
+        self._check(self.lib.dll.acNodeReduceScal(self.handle, stream, _rt(rtype), _vh(handle), ctypes.byref(out)),
+                    "acNodeReduceScal")
+        return out.value
+
+    def reduce_vec(self, rtype, a, b, c, stream=abi.STREAM_DEFAULT):
+        out = self.lib.real()
+        self._check(self.lib.dll.acNodeReduceVec(self.handle, stream, _rt(rtype), _vh(a), _vh(b), _vh(c),
+                                                 ctypes.byref(out)), "acNodeReduceVec")
+        return out.value

@@ -25,7 +25,7 @@ INCLUDE = os.path.join(REPO, "include")
 
 HIP_SOURCES = ["device.hip", "solve.hip", "boundconds.hip", "reductions.hip", "init.hip", "halo.hip",
                "ensemble.hip"]
-HOST_SOURCES = ["host.cc", "legacy.cc", "simulation.cc", "fortran.cc"]
+HOST_SOURCES = ["host.cc", "legacy.cc", "simulation.cc", "fortran.cc", "node.cc"]
 ARCH = "gfx950"
 
 

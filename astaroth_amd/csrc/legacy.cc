@@ -11,7 +11,7 @@
 //   acIntegrateGBC = 3 x {general fill by the face types of `config`, substep, swap}
 //                    (node.cc:808-880), acBoundcondStepGBC = the general fill alone
 //                    (node.cc:923-930), both with the corrected fill of boundconds.hip.
-// The Node handle itself (acGetNode) is not provided.
+// acGetNode returns a Node view of the Device (the Node interface of node.cc over it).
 #include "astaroth.h"
 #include "astaroth_ext.h"
 
@@ -19,9 +19,13 @@
 #include <cstdio>
 #include <hip/hip_runtime_api.h>
 
+// node.cc: a Node that wraps an existing Device without owning it
+Node ac_node_wrap(Device device, const AcMeshInfo& info);
+
 namespace {
 
 Device g_device        = nullptr;
+Node g_node            = nullptr; // view of g_device
 AcMeshInfo g_info      = {};
 
 bool
@@ -64,12 +68,22 @@ AcResult
 acInit(const AcMeshInfo mesh_info)
 {
     if (g_device) {
+        acNodeDestroy(g_node); // the view only
         acDeviceDestroy(g_device);
         g_device = nullptr;
+        g_node   = nullptr;
     }
     g_info = mesh_info;
     acHostUpdateBuiltinParams(&g_info);
-    return acDeviceCreate(0, g_info, &g_device);
+    if (acDeviceCreate(0, g_info, &g_device) != AC_SUCCESS)
+        return AC_FAILURE;
+    g_node = ac_node_wrap(g_device, g_info);
+    if (!g_node) {
+        acDeviceDestroy(g_device);
+        g_device = nullptr;
+        return AC_FAILURE;
+    }
+    return AC_SUCCESS;
 }
 
 AcResult
@@ -77,9 +91,19 @@ acQuit(void)
 {
     if (!ready("acQuit"))
         return AC_FAILURE;
+    acNodeDestroy(g_node); // the view only
     const AcResult r = acDeviceDestroy(g_device);
     g_device         = nullptr;
+    g_node           = nullptr;
     return r;
+}
+
+Node
+acGetNode(void)
+{
+    if (!ready("acGetNode"))
+        return nullptr;
+    return g_node;
 }
 
 AcResult

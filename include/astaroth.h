@@ -8,9 +8,9 @@
  * against the reference Device interface compile and link against this header and
  * libastaroth_fp32.so / libastaroth_fp64.so unchanged (AC_DOUBLE_PRECISION selects which).
  *
- * The reference's Node and Grid layers are multi-GPU wrappers over this interface and are
- * not declared here. Its legacy single-call interface (acInit ... acQuit, :208-283) is
- * provided over one Device (one GPU); acGetNode is not, since there is no Node.
+ * The reference's Node layer (:355-500), a wrapper that splits the box over the GPUs of a node,
+ * and its legacy single-call interface (acInit ... acGetNode, :208-283) are provided over one
+ * Device (one GPU). The Grid layer (MPI over several nodes) is not.
  */
 #pragma once
 #include <float.h>  /* FLT_EPSILON, etc */
@@ -108,6 +108,23 @@ typedef struct {
 
 /* Opaque handle to one GPU's state (reference include/astaroth.h:122). */
 typedef struct device_s* Device;
+
+/* Opaque handle to a Node (reference :126): here one Device. */
+typedef struct node_s* Node;
+
+/* Grid dimensions with (m) and without (n) the ghost zones (reference :131-134). */
+typedef struct {
+    int3 m;
+    int3 n;
+} GridDims;
+
+/* What acNodeQueryDeviceConfiguration reports (reference :136-142). */
+typedef struct {
+    int num_devices;
+    Device* devices;
+    GridDims grid;
+    GridDims subgrid;
+} DeviceConfiguration;
 
 #ifdef __cplusplus
 extern "C" {
@@ -320,6 +337,77 @@ AcResult acIntegrateStepWithOffset(const int isubstep, const AcReal dt, const in
 AcResult acSynchronize(void);
 AcResult acLoadWithOffset(const AcMesh host_mesh, const int3 src, const int num_vertices);
 int acGetNumDevicesPerNode(void);
+/* The Node view of the legacy interface's Device (valid between acInit and acQuit). */
+Node acGetNode(void);
+
+/* ---- Node interface (ref :355-500, src/core/node.cc) over one Device ----
+ * acNodeCreate creates one Device on GPU 0 for the whole grid (its subgrid is the grid, so
+ * acNodeQueryDeviceConfiguration reports num_devices = 1 and subgrid = grid) and every call is
+ * the matching Device call, with these specifics:
+ * - the *WithOffset load/store forms copy from `src` to `dst` as the Device forms do (the
+ *   reference Node uses `src` for both, node.cc:452-455, 549-552);
+ * - acNodeIntegrateSubstep clips the region to the interior and does nothing when it is empty;
+ * - acNodeIntegrate / acNodeIntegrateGBC: per substep the inner region [2 NGHOST, n) is
+ *   integrated on STREAM_16 while the ghost shell is filled (periodic / by the face types of
+ *   `config`) on STREAM_0, then the six outer slabs on STREAM_0..5, then the buffers swap; the
+ *   result is that of 3 x {fill, substep over the interior, swap} to rounding (the launches
+ *   start their z-march columns at other planes); synchronous;
+ * - acNodeReduce* finalise RMS, RMS_EXP and ALFVEN_RMS as sqrt(sum / n_x n_y n_z);
+ * - acNodePrintInfo and acNodeAutoOptimize are the Device's (the reference returns AC_FAILURE).
+ * The load, store, set and reduction calls first synchronise their stream, as the reference
+ * Node does. */
+AcResult acNodeCreate(const int id, const AcMeshInfo node_config, Node* node);
+AcResult acNodeDestroy(Node node);
+AcResult acNodePrintInfo(const Node node);
+AcResult acNodeQueryDeviceConfiguration(const Node node, DeviceConfiguration* config);
+AcResult acNodeAutoOptimize(const Node node);
+AcResult acNodeSynchronizeStream(const Node node, const Stream stream);
+AcResult acNodeSynchronizeVertexBuffer(const Node node, const Stream stream,
+                                       const VertexBufferHandle vtxbuf_handle);
+AcResult acNodeSynchronizeMesh(const Node node, const Stream stream);
+AcResult acNodeSwapBuffers(const Node node);
+AcResult acNodeLoadConstant(const Node node, const Stream stream, const AcRealParam param,
+                            const AcReal value);
+AcResult acNodeLoadVertexBufferWithOffset(const Node node, const Stream stream,
+                                          const AcMesh host_mesh,
+                                          const VertexBufferHandle vtxbuf_handle, const int3 src,
+                                          const int3 dst, const int num_vertices);
+AcResult acNodeLoadMeshWithOffset(const Node node, const Stream stream, const AcMesh host_mesh,
+                                  const int3 src, const int3 dst, const int num_vertices);
+AcResult acNodeLoadVertexBuffer(const Node node, const Stream stream, const AcMesh host_mesh,
+                                const VertexBufferHandle vtxbuf_handle);
+AcResult acNodeLoadMesh(const Node node, const Stream stream, const AcMesh host_mesh);
+AcResult acNodeSetVertexBuffer(const Node node, const Stream stream,
+                               const VertexBufferHandle handle, const AcReal value);
+AcResult acNodeStoreVertexBufferWithOffset(const Node node, const Stream stream,
+                                           const VertexBufferHandle vtxbuf_handle, const int3 src,
+                                           const int3 dst, const int num_vertices,
+                                           AcMesh* host_mesh);
+AcResult acNodeStoreMeshWithOffset(const Node node, const Stream stream, const int3 src,
+                                   const int3 dst, const int num_vertices, AcMesh* host_mesh);
+AcResult acNodeStoreVertexBuffer(const Node node, const Stream stream,
+                                 const VertexBufferHandle vtxbuf_handle, AcMesh* host_mesh);
+AcResult acNodeStoreMesh(const Node node, const Stream stream, AcMesh* host_mesh);
+AcResult acNodeIntegrateSubstep(const Node node, const Stream stream, const int step_number,
+                                const int3 start, const int3 end, const AcReal dt);
+AcResult acNodeIntegrate(const Node node, const AcReal dt);
+AcResult acNodeIntegrateGBC(const Node node, const AcMeshInfo config, const AcReal dt);
+AcResult acNodePeriodicBoundcondStep(const Node node, const Stream stream,
+                                     const VertexBufferHandle vtxbuf_handle);
+AcResult acNodePeriodicBoundconds(const Node node, const Stream stream);
+AcResult acNodeGeneralBoundcondStep(const Node node, const Stream stream,
+                                    const VertexBufferHandle vtxbuf_handle,
+                                    const AcMeshInfo config);
+AcResult acNodeGeneralBoundconds(const Node node, const Stream stream, const AcMeshInfo config);
+AcResult acNodeReduceScal(const Node node, const Stream stream, const ReductionType rtype,
+                          const VertexBufferHandle vtxbuf_handle, AcReal* result);
+AcResult acNodeReduceVec(const Node node, const Stream stream_type, const ReductionType rtype,
+                         const VertexBufferHandle vtxbuf0, const VertexBufferHandle vtxbuf1,
+                         const VertexBufferHandle vtxbuf2, AcReal* result);
+AcResult acNodeReduceVecScal(const Node node, const Stream stream_type, const ReductionType rtype,
+                             const VertexBufferHandle vtxbuf0, const VertexBufferHandle vtxbuf1,
+                             const VertexBufferHandle vtxbuf2, const VertexBufferHandle vtxbuf3,
+                             AcReal* result);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -1,6 +1,7 @@
 /* A C program in the style of the reference's samples/ctest, written against include/astaroth.h
- * (Device interface and the legacy ac* interface). The CPU test suite compiles and links it; the
- * GPU suite runs it and compares its printed numbers with the Device-level results. */
+ * (the legacy ac* interface, or with a second argument "node" the same sequence through the Node
+ * interface). The CPU test suite compiles and links it; the GPU suite runs it and compares its
+ * printed numbers with the Device-level results. */
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -45,20 +46,46 @@ main(int argc, char** argv)
         printf("no device\n");
         return 0;
     }
-    if (acInit(info) != AC_SUCCESS)
-        return 1;
-    acLoad(mesh);
-    acBoundcondStep();
-    for (int s = 0; s < 2; ++s)
-        acIntegrate((AcReal)1e-3);
-    acBoundcondStep();
-    acStore(&mesh);
-    const AcReal umax  = acReduceVec(RTYPE_MAX, VTXBUF_UUX, VTXBUF_UUY, VTXBUF_UUZ);
-    const AcReal urms  = acReduceVec(RTYPE_RMS, VTXBUF_UUX, VTXBUF_UUY, VTXBUF_UUZ);
-    const AcReal lrmin = acReduceScal(RTYPE_MIN, VTXBUF_LNRHO);
-    const AcReal arms  = acReduceVecScal(RTYPE_ALFVEN_RMS, VTXBUF_AX, VTXBUF_AY, VTXBUF_AZ,
-                                        VTXBUF_LNRHO);
-    acQuit();
+    AcReal umax, urms, lrmin, arms;
+    if (argc > 2 && argv[2][0] == 'n') { /* Node interface */
+        Node node;
+        if (acNodeCreate(0, info, &node) != AC_SUCCESS)
+            return 1;
+        DeviceConfiguration dc;
+        if (acNodeQueryDeviceConfiguration(node, &dc) != AC_SUCCESS || dc.num_devices != 1 ||
+            dc.subgrid.n.z != n)
+            return 1;
+        acNodeLoadMesh(node, STREAM_DEFAULT, mesh);
+        acNodePeriodicBoundconds(node, STREAM_DEFAULT);
+        for (int s = 0; s < 2; ++s)
+            acNodeIntegrate(node, (AcReal)1e-3);
+        acNodePeriodicBoundconds(node, STREAM_DEFAULT);
+        acNodeStoreMesh(node, STREAM_DEFAULT, &mesh);
+        acNodeSynchronizeStream(node, STREAM_ALL);
+        acNodeReduceVec(node, STREAM_DEFAULT, RTYPE_MAX, VTXBUF_UUX, VTXBUF_UUY, VTXBUF_UUZ, &umax);
+        acNodeReduceVec(node, STREAM_DEFAULT, RTYPE_RMS, VTXBUF_UUX, VTXBUF_UUY, VTXBUF_UUZ, &urms);
+        acNodeReduceScal(node, STREAM_DEFAULT, RTYPE_MIN, VTXBUF_LNRHO, &lrmin);
+        acNodeReduceVecScal(node, STREAM_DEFAULT, RTYPE_ALFVEN_RMS, VTXBUF_AX, VTXBUF_AY,
+                            VTXBUF_AZ, VTXBUF_LNRHO, &arms);
+        acNodeDestroy(node);
+    }
+    else {
+        if (acInit(info) != AC_SUCCESS)
+            return 1;
+        acLoad(mesh);
+        acBoundcondStep();
+        for (int s = 0; s < 2; ++s)
+            acIntegrate((AcReal)1e-3);
+        acBoundcondStep();
+        acStore(&mesh);
+        if (!acGetNode())
+            return 1;
+        umax  = acReduceVec(RTYPE_MAX, VTXBUF_UUX, VTXBUF_UUY, VTXBUF_UUZ);
+        urms  = acReduceVec(RTYPE_RMS, VTXBUF_UUX, VTXBUF_UUY, VTXBUF_UUZ);
+        lrmin = acReduceScal(RTYPE_MIN, VTXBUF_LNRHO);
+        arms  = acReduceVecScal(RTYPE_ALFVEN_RMS, VTXBUF_AX, VTXBUF_AY, VTXBUF_AZ, VTXBUF_LNRHO);
+        acQuit();
+    }
     printf("%.17g %.17g %.17g %.17g %.17g\n", (double)umax, (double)urms, (double)lrmin,
            (double)arms, (double)mesh.vertex_buffer[VTXBUF_UUX][acVertexBufferIdx(5, 6, 7, info)]);
     acHostMeshDestroy(&mesh);

@@ -1,10 +1,11 @@
-"""The legacy single-call interface (acInit ... acQuit, reference include/astaroth.h:208-283)
-from a C program built against include/ with the system C compiler.
+"""The legacy single-call interface (acInit ... acQuit, reference include/astaroth.h:208-283) and
+the Node interface (:355-500) from a C program built against include/ with the system C compiler.
 
 CPU: the program compiles and links against both libraries (FP32 and FP64 headers).
 GPU: it runs, and its numbers equal the same sequence driven through the Device interface:
-2 x acIntegrate == 2 x acDeviceIntegrateStep (bitwise), and acReduce* == the Device reductions
-finalised as sqrt(sum / N) for RMS types (node.cc:932-957).
+2 x acIntegrate == 2 x acNodeIntegrate == 2 x acDeviceIntegrateStep (bitwise), and acReduce* /
+acNodeReduce* == the Device reductions finalised as sqrt(sum / N) for RMS types
+(node.cc:932-957).
 """
 import ctypes
 import math
@@ -55,6 +56,13 @@ def test_legacy_interface_matches_device_interface(tmp_path, double):
     assert out.returncode == 0, out.stderr
     got = [float(v) for v in out.stdout.split()]
     assert len(got) == 5, out.stdout
+    # the Node interface (inner region and outer slabs on separate streams) prints the same
+    # numbers to rounding (its launches start their z-march columns at other planes)
+    node = subprocess.run([exe, str(n), "node"], capture_output=True, text=True, timeout=120,
+                          env=env)
+    assert node.returncode == 0, node.stderr
+    got_node = [float(v) for v in node.stdout.split()]
+    assert got_node == pytest.approx(got, rel=1e-5 if not double else 1e-12)
 
     # the same sequence through the Device interface (same host initial state)
     lib = Library(double)
