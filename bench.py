@@ -30,8 +30,16 @@ sys.path.insert(0, REPO)
 
 METRIC = "cell-updates/s per RK3 step at 512^3, FP32 vs FP64, + % of roofline"
 HBM_PEAK_GBS = 8000.0
-VALU_FP32_PEAK_TFLOPS = 157.3  # vendor figure (counts packed FP32; packed math is not used here)
-VALU_FP64_PEAK_TFLOPS = 78.6
+# Vector peak in use for both precisions: packed FP32 is compiled out, and unpacked FP32 runs at
+# the FP64 rate (the vendor's 157.3 TFLOP/s FP32 figure counts packed math).
+VALU_PEAK_TFLOPS = 78.6
+# Floating-point operations per cell of one substep of the reference formulation that the eight
+# rates depend on (adds + multiplies + divides), counted by oracle/opcount.cc ("live" lines;
+# tests/test_oracle_opcount.py checks these constants against it): the right-hand side, its
+# exponentials (not in the FLOP figure), and the RK3 update of substeps 0, 1 and 2.
+FLOP_RHS = 552 + 421 + 9
+EXP_RHS = 7
+FLOP_UPDATE = (24, 48, 48)
 
 
 from astaroth_amd.hiprt import Hip  # noqa: E402  (events on the library's own HIP runtime)
@@ -125,6 +133,28 @@ def run_gpu(double: bool, n: int, steps: int, warmup: int, dt: float, autotune: 
         # ghost shell read + written once per field (fill), or read, packed, unpacked (pack)
         halo_alg_bytes=float((n + 6) ** 3 - n ** 3) * 8 * s * (2 if halo == "fill" else 4),
     )
+
+
+def valu_roofline(r, n):
+    """Vector-ALU side of the solve launch: algorithmic FLOP (mean over the three substeps) per
+    launch / its mean duration, against the unpacked vector peak."""
+    flop_cell = FLOP_RHS + sum(FLOP_UPDATE) / 3.0
+    tflops = n ** 3 * flop_cell / (r["solve_ms_mean"] * 1e-3) / 1e12
+    return {"achieved": tflops, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tflops / VALU_PEAK_TFLOPS, "flop_per_cell_substep": flop_cell,
+            "exp_per_cell_substep": EXP_RHS, "flop_source": "oracle/opcount.cc (live operations)"}
+
+
+def ceilings(r, n, double):
+    """Step-rate ceilings (cell-updates/s) set by HBM (64 n^3 s bytes per RK3 step) and by the
+    vector ALUs (3 x right-hand side + the three updates per cell and step), and the measured
+    rate's fraction of the lower one."""
+    s = 8 if double else 4
+    hbm = HBM_PEAK_GBS * 1e9 / (64 * s)
+    valu = VALU_PEAK_TFLOPS * 1e12 / (3 * FLOP_RHS + sum(FLOP_UPDATE))
+    low = min(hbm, valu)
+    return {"hbm": hbm, "valu": valu, "binding": "hbm" if hbm <= valu else "valu",
+            "frac_of_binding": r["value"] / low, "unit": "cell-updates/s"}
 
 
 def step_diagnostics(dev):
@@ -333,6 +363,8 @@ def main(argv=None):
             "traffic_source": traffic_src,
             "alg_bytes_per_launch": r["alg_bytes_per_launch"],
             "solve_ms_per_launch": r["solve_ms_mean"],
+            "valu": valu_roofline(r, a.n),
+            "ceiling": ceilings(r, a.n, main_prec == "fp64"),
         },
     }
     if "fp64" in res and "fp32" in res:
@@ -340,6 +372,7 @@ def main(argv=None):
         line["fp64"] = {"value": f64["value"], "ms_per_step": f64["ms_per_step"],
                         "achieved_gbs": f64["achieved_gbs"], "frac": f64["achieved_gbs"] / HBM_PEAK_GBS,
                         "kernel_config": f64["kernel_config"], "solve_ms_per_launch": f64["solve_ms_mean"],
+                        "valu": valu_roofline(f64, a.n), "ceiling": ceilings(f64, a.n, True),
                         "step": "3 x {solve+RK3 substep reading the periodic images of the ghost "
                                 "cells, swap} (the default FP64 configuration fills no ghosts)"}
         line["fp32_over_fp64"] = r["value"] / f64["value"]
