@@ -149,6 +149,8 @@ class Library:
             "acLoad": (i, [M]),
             "acReduceScal": (R, [i, i]),
             "acDeviceIntegrateStep": (i, [vp, i, R]),
+            "acDeviceIntegrateSubstepPeriodic": (i, [vp, i, i, R]),
+            "acDeviceSubstepReadsPeriodicImages": (i, [vp]),
             "acDeviceIntegrateStepGraph": (i, [vp, i, R]),
             "acKernelConfigCount": (i, []),
             "acKernelConfigGet": (i, [i, ctypes.POINTER(abi.KernelConfig)]),

@@ -1013,6 +1013,36 @@ step_reads_images(const Device d)
 }
 
 AcResult
+acDeviceIntegrateSubstepPeriodic(const Device device, const Stream stream, const int step_number,
+                                 const AcReal dt)
+{
+    const int3 nmin = mk3(NGHOST, NGHOST, NGHOST);
+    const int3 nmax = mk3(nx_of(device) + NGHOST, ny_of(device) + NGHOST, nz_of(device) + NGHOST);
+    if (!step_reads_images(device)) {
+        const int3 m = mk3(nx_of(device) + 6, ny_of(device) + 6, nz_of(device) + 6);
+        if (acDevicePeriodicBoundconds(device, stream, mk3(0, 0, 0), m) != AC_SUCCESS)
+            return AC_FAILURE;
+        return acDeviceIntegrateSubstep(device, stream, step_number, nmin, nmax, dt);
+    }
+    if (step_number < 0 || step_number > 2)
+        AC_FATAL("acDeviceIntegrateSubstepPeriodic: step_number %d is not 0, 1 or 2", step_number);
+    if (!stream_ok(device, stream) ||
+        acDeviceLoadScalarUniform(device, stream, AC_dt, dt) != AC_SUCCESS)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(device->id));
+    SubstepArgs a = make_args(device, device->vba.in, device->vba.out, step_number, nmin, nmax, dt);
+    a.wrap        = 1;
+    AC_HIP_CHECK(launch_substep(device->kernel_config, a, device->streams[stream]));
+    return AC_SUCCESS;
+}
+
+int
+acDeviceSubstepReadsPeriodicImages(const Device device)
+{
+    return step_reads_images(device) ? 1 : 0;
+}
+
+AcResult
 acDeviceIntegrateStep(const Device device, const Stream stream, const AcReal dt)
 {
     const int3 zero = mk3(0, 0, 0);
@@ -1022,15 +1052,9 @@ acDeviceIntegrateStep(const Device device, const Stream stream, const AcReal dt)
     if (step_reads_images(device)) {
         // The build reads the periodic images of the ghost cells itself: the substeps need no
         // fill between them and compute what they compute after one.
-        if (!stream_ok(device, stream) ||
-            acDeviceLoadScalarUniform(device, stream, AC_dt, dt) != AC_SUCCESS)
-            return AC_FAILURE;
-        AC_HIP_CHECK(hipSetDevice(device->id));
-        hipStream_t hs = device->streams[stream];
         for (int s = 0; s < 3; ++s) {
-            SubstepArgs a = make_args(device, device->vba.in, device->vba.out, s, nmin, nmax, dt);
-            a.wrap        = 1;
-            AC_HIP_CHECK(launch_substep(device->kernel_config, a, hs));
+            if (acDeviceIntegrateSubstepPeriodic(device, stream, s, dt) != AC_SUCCESS)
+                return AC_FAILURE;
             acDeviceSwapBuffers(device);
         }
         return AC_SUCCESS;

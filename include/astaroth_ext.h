@@ -36,6 +36,13 @@ AcResult acDeviceReduceVecScalDouble(const Device device, const Stream stream,
  * ghost cells are to be filled before they are read. */
 AcResult acDeviceIntegrateStep(const Device device, const Stream stream, const AcReal dt);
 
+/* One substep of acDeviceIntegrateStep: over the whole interior of a periodic box, reading the
+ * periodic images of the ghost cells where the step does (acDeviceSubstepReadsPeriodicImages
+ * returns 1), otherwise after a periodic fill of `in`. No swap. */
+AcResult acDeviceIntegrateSubstepPeriodic(const Device device, const Stream stream,
+                                          const int step_number, const AcReal dt);
+int acDeviceSubstepReadsPeriodicImages(const Device device);
+
 /* The step with general boundaries (acNodeIntegrateGBC, node.cc:808-880, on one Device):
  * 3 x {acDeviceGeneralBoundconds over the whole box with the face types of `config`, substep,
  * swap}; acDeviceIntegrateStep when every face is periodic. */

@@ -391,8 +391,8 @@ def test_step_reads_periodic_images(lib, zd):
     before each substep). In each case the step's interior equals, bit for bit, that of the fill
     sequence made by hand (3 x {acDevicePeriodicBoundconds, acDeviceIntegrateSubstep, swap}),
     with every ghost cell of both buffer sets NaN (a ghost read would carry a NaN into the
-    interior); the same for the recorded step and for the general-boundary step with periodic
-    faces."""
+    interior); the same for the recorded step, for the general-boundary step with periodic faces
+    and for three acDeviceIntegrateSubstepPeriodic calls."""
     nx, ny, nz = 21, 18, 15
     dt = 1e-3
     info = lib.config(n=nx, ny=ny, nz=nz)
@@ -414,7 +414,8 @@ def test_step_reads_periodic_images(lib, zd):
         return dev, mesh
 
     lean = (8 if lib.double else 19, 16, 16, zd)
-    for config, modes in ((lean, ("eager", "graph", "gbc")), ((2, 32, 4, zd), ("eager",))):
+    for config, modes in ((lean, ("eager", "graph", "gbc", "substeps")),
+                          ((2, 32, 4, zd), ("eager", "substeps"))):
         dev, mesh = loaded(config)
         with dev:
             for s in range(3):
@@ -429,6 +430,14 @@ def test_step_reads_periodic_images(lib, zd):
             with dev:
                 if mode == "gbc":
                     dev.integrate_step_gbc(info, lib.real(dt))
+                elif mode == "substeps":  # acDeviceIntegrateSubstepPeriodic, one substep at a time
+                    reads = lib.dll.acDeviceSubstepReadsPeriodicImages(dev.handle)
+                    assert reads == (1 if (lib.double and config[0] == 8) else 0)
+                    for s in range(3):
+                        assert lib.dll.acDeviceIntegrateSubstepPeriodic(
+                            dev.handle, abi.STREAM_DEFAULT, s, lib.real(dt)) == abi.AC_SUCCESS
+                        dev.swap_buffers()
+                    dev.synchronize()
                 else:
                     dev.integrate_step(dt, graph=(mode == "graph"))
                 out = mesh.copy()
