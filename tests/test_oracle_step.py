@@ -230,3 +230,33 @@ def test_gpu_form_substep_only_writes_region():
         mask[box] = False
         assert np.array_equal(out[q][mask], prev[q][mask])
         assert not np.array_equal(out[q][box], prev[q][box])
+
+
+def test_dependency_cone_slice_equals_full_box():
+    """The slice method of tests/test_gpu_precision_slice.py, on the CPU: a slice after k RK3
+    steps computed from the block within 9 k cells of it (30 substeps of the two-buffer form over
+    regions shrinking by 3 per side) equals, bit for bit, the same slice of the whole periodic box
+    stepped with the periodic fill."""
+    from astaroth_amd.api import Library, real_params_double
+    from astaroth_amd.initial import mhd_fields
+
+    n, steps, dt = 40, 2, 1e-3
+    reach = 9 * steps
+    s0, sw = 18, 3
+    info = Library(True).config(n=n)
+    rp = real_params_double(info)
+    full = mhd_fields(n, n, n, np.float64, config_id=3, perturb_sin2z=True)
+    b0, bw = s0 + 3 - reach - 3, sw + 2 * reach + 6
+    assert b0 >= 0 and b0 + bw <= n + 6  # the block lies inside the stored box
+    a = [np.ascontiguousarray(f[b0:b0 + bw, b0:b0 + bw, b0:b0 + bw]) for f in full]
+    b = [np.zeros_like(f) for f in a]
+    lo = 3 + reach
+    for j in range(3 * steps):
+        m = 3 * (3 * steps - 1 - j)
+        pyoracle.substep_gpu_form(a, b, rp, j % 3, dt, (lo - m,) * 3, (lo + sw + m,) * 3)
+        a, b = b, a
+    pyoracle.integrate_steps_gpu_form(full, rp, dt, steps)
+    for q in range(8):
+        got = a[q][lo:lo + sw, lo:lo + sw, lo:lo + sw]
+        ref = full[q][s0 + 3:s0 + 3 + sw, s0 + 3:s0 + 3 + sw, s0 + 3:s0 + 3 + sw]
+        assert np.array_equal(got, ref), q
