@@ -430,7 +430,8 @@ def run_workload(a, kcfg, hip):
                     roofline={"bound": "hbm", "kernel": "solve (fused MHD stencil + RK3 substep)",
                               "achieved": r["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": r["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
-                              "solve_ms_per_launch": r["solve_ms_mean"]},
+                              "solve_ms_per_launch": r["solve_ms_mean"],
+                              "valu": valu_roofline(r, n), "ceiling": ceilings(r, n, False)},
                     halo={"bytes_per_exchange": r["halo_alg_bytes"], "ms_per_exchange": r["halo_ms_mean"],
                           "achieved_gbs": r["halo_alg_bytes"] / (r["halo_ms_mean"] * 1e-3) / 1e9},
                     diagnostics_last_step=r["reductions"][-1] if r["reductions"] else None)
