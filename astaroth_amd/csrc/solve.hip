@@ -31,9 +31,15 @@
 //  variants 7-12: the lean z-march with a 5-slot ring (7: one barrier per plane), or loading
 //      the previous stage before the next plane and storing each output plane at the start of
 //      the next step (8: 4 slots, the FP64 default; 9: 5 slots), and its two-waves-per-SIMD
-//      builds on 16 x 16 tiles (10: deferred stores; 11: previous stage first; 12: both, the
-//      FP32 default).
+//      builds on 16 x 16 tiles (10: deferred stores; 11: previous stage first; 12: both).
+//  variant 13: variant 12 with an 8-phase plane loop (compile-time ring slots); 16: variant 12
+//      with a 5-slot ring.
+//  variants 17, 19, 22: variants 16, 12 and 10 with pinned LDS windows (every stencil read of a
+//      field's plane through one base register); 19 is the FP32 default.
+//  variant 23: variant 17 loading each plane straight into its ring slot (direct-to-LDS loads).
 //  variants 5, 6: the lean z-march with a cell-major LDS ring (FP32; kept for comparison).
+//  The FP64 default also has a build that reads the periodic images of the ghost cells (WRAP),
+//  used by the whole-step entry points of a periodic box.
 #include "physics.h"
 
 namespace ac {
