@@ -151,6 +151,7 @@ class Library:
             "acDeviceIntegrateStep": (i, [vp, i, R]),
             "acDeviceIntegrateSubstepPeriodic": (i, [vp, i, i, R]),
             "acDeviceSubstepReadsPeriodicImages": (i, [vp]),
+            "acDeviceSetReadPeriodicImages": (i, [vp, i]),
             "acDeviceIntegrateStepGraph": (i, [vp, i, R]),
             "acKernelConfigCount": (i, []),
             "acKernelConfigGet": (i, [i, ctypes.POINTER(abi.KernelConfig)]),

@@ -96,6 +96,9 @@ hipError_t launch_substep(const AcKernelConfig& config, const SubstepArgs& args,
 bool kernel_config_supported(const AcKernelConfig& config);
 // True when `config` has a build that reads periodic images (SubstepArgs::wrap = 1).
 bool kernel_config_reads_images(const AcKernelConfig& config);
+// Whether a periodic step of an n_x x n_y x n_z box reads the images rather than filling the ghost
+// cells, before the autotuner has measured both (the measured choice for the default builds).
+bool default_reads_images(int nx, int ny, int nz);
 // Batched solve (persistent lean z-march over a work list): `boxes` and `items` are device
 // arrays; tile shapes 32x4, 16x8, 16x16.
 bool batch_tile_supported(int tile_x, int tile_y);
@@ -164,6 +167,9 @@ struct device_s {
     double* diag_result;
     double* diag_result_host;
     AcKernelConfig kernel_config;
+    // periodic steps read the ghost cells' images where the configuration has such a build
+    // (default by size, measured by acDeviceAutoOptimize, set by acDeviceSetReadPeriodicImages)
+    bool read_images;
     // compensated integration (acDeviceSetCompensatedIntegration): comp[0] pairs with vba.in,
     // comp[1] with vba.out; null when off
     AcReal* comp[2][ac::NF];

@@ -76,11 +76,12 @@ invalidate_graphs(Device d)
     }
 }
 
-// Autotuner results per (nx, ny, nz), shared by every Device of the process.
-std::map<std::tuple<int, int, int>, AcKernelConfig>&
+// Autotuner results per (nx, ny, nz), shared by every Device of the process: the configuration
+// and whether periodic steps read the ghost cells' images.
+std::map<std::tuple<int, int, int>, std::pair<AcKernelConfig, bool>>&
 tune_cache()
 {
-    static std::map<std::tuple<int, int, int>, AcKernelConfig> cache;
+    static std::map<std::tuple<int, int, int>, std::pair<AcKernelConfig, bool>> cache;
     return cache;
 }
 
@@ -121,12 +122,13 @@ region_ok(const Device d, int3 start, int3 end)
 }
 
 float
-time_config(Device d, const AcKernelConfig& c, int iters)
+time_config(Device d, const AcKernelConfig& c, int iters, bool wrap = false)
 {
     const int3 start = mk3(NGHOST, NGHOST, NGHOST);
     const int3 end   = mk3(nx_of(d) + NGHOST, ny_of(d) + NGHOST, nz_of(d) + NGHOST);
-    const SubstepArgs a =
+    SubstepArgs a =
         make_args(d, d->vba.in, d->vba.out, 2, start, end, uniforms_of(d).real_params[AC_dt]);
+    a.wrap = wrap; // the build that reads the ghost cells' periodic images
     hipStream_t s = d->streams[STREAM_DEFAULT];
     if (launch_substep(c, a, s) != hipSuccess) { // warm-up, also rejects unusable shapes
         (void)hipGetLastError();
@@ -139,6 +141,30 @@ time_config(Device d, const AcKernelConfig& c, int iters)
     for (int i = 0; i < iters; ++i)
         if (launch_substep(c, a, s) != hipSuccess)
             break;
+    AC_HIP_CHECK(hipEventRecord(t1, s));
+    AC_HIP_CHECK(hipEventSynchronize(t1));
+    float ms = 0;
+    AC_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+    AC_HIP_CHECK(hipEventDestroy(t0));
+    AC_HIP_CHECK(hipEventDestroy(t1));
+    return ms / (iters > 0 ? iters : 1);
+}
+
+// Mean time of one periodic fill of all fields, timed on the `out` buffers (scratch while tuning,
+// as the timed launches' output is) so that `in` is unchanged.
+float
+time_fill(Device d, int iters)
+{
+    const int nx = nx_of(d), ny = ny_of(d), nz = nz_of(d);
+    const int3 zero = mk3(0, 0, 0), m = mk3(nx + 6, ny + 6, nz + 6);
+    hipStream_t s   = d->streams[STREAM_DEFAULT];
+    hipEvent_t t0, t1;
+    AC_HIP_CHECK(hipEventCreate(&t0));
+    AC_HIP_CHECK(hipEventCreate(&t1));
+    AC_HIP_CHECK(launch_periodic_fill(d->vba.out, NF, nx, ny, nz, zero, m, s));
+    AC_HIP_CHECK(hipEventRecord(t0, s));
+    for (int i = 0; i < iters; ++i)
+        AC_HIP_CHECK(launch_periodic_fill(d->vba.out, NF, nx, ny, nz, zero, m, s));
     AC_HIP_CHECK(hipEventRecord(t1, s));
     AC_HIP_CHECK(hipEventSynchronize(t1));
     float ms = 0;
@@ -477,6 +503,7 @@ acDeviceCreate(const int id, const AcMeshInfo config, Device* device_handle)
     acDeviceLoadDefaultUniforms(d);
     acDeviceLoadMeshInfo(d, config);
     d->kernel_config = default_kernel_config(nx, ny, nz);
+    d->read_images   = default_reads_images(nx, ny, nz);
     *device_handle   = d;
 
     const char* tune = getenv("AC_AUTOTUNE");
@@ -1007,9 +1034,20 @@ acDeviceRunMPITest(void)
 static bool
 step_reads_images(const Device d)
 {
-    if (d->comp[0][0] != nullptr || nx_of(d) < NGHOST || ny_of(d) < NGHOST || nz_of(d) < NGHOST)
+    if (!d->read_images || d->comp[0][0] != nullptr || nx_of(d) < NGHOST || ny_of(d) < NGHOST ||
+        nz_of(d) < NGHOST)
         return false;
     return kernel_config_reads_images(d->kernel_config);
+}
+
+AcResult
+acDeviceSetReadPeriodicImages(const Device device, const int enable)
+{
+    if (enable && !kernel_config_reads_images(device->kernel_config))
+        return AC_FAILURE;
+    device->read_images = enable != 0;
+    invalidate_graphs(device);
+    return AC_SUCCESS;
 }
 
 AcResult
@@ -1196,7 +1234,8 @@ acDeviceAutoOptimize(const Device device)
     {
         auto it = tune_cache().find(key);
         if (it != tune_cache().end()) {
-            device->kernel_config = it->second;
+            device->kernel_config = it->second.first;
+            device->read_images   = it->second.second;
             invalidate_graphs(device);
             return AC_SUCCESS;
         }
@@ -1232,8 +1271,28 @@ acDeviceAutoOptimize(const Device device)
     }
     if (best.variant == 1 && best.z_depth == 0)
         best.z_depth = nz_of(device);
+    // A periodic substep costs a fill plus the launch, or one launch of a build that reads the
+    // ghost cells' periodic images (the defaults have one): keep the cheaper of the fastest
+    // configuration with its fill and the image-reading builds of it and of the default.
+    bool images = false;
+    if (nx_of(device) >= NGHOST && ny_of(device) >= NGHOST && nz_of(device) >= NGHOST) {
+        float best_substep = best_ms + time_fill(device, iters);
+        const AcKernelConfig candidates[2] = {
+            best, default_kernel_config(nx_of(device), ny_of(device), nz_of(device))};
+        for (const AcKernelConfig& c : candidates) {
+            if (!kernel_config_reads_images(c))
+                continue;
+            const float ms = time_config(device, c, iters, true);
+            if (ms > 0 && ms < best_substep) {
+                best_substep = ms;
+                best         = c;
+                images       = true;
+            }
+        }
+    }
     device->kernel_config = best;
-    tune_cache()[key]     = best;
+    device->read_images   = images;
+    tune_cache()[key]     = std::make_pair(best, images);
     invalidate_graphs(device);
     return AC_SUCCESS;
 }

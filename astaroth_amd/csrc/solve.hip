@@ -38,8 +38,8 @@
 //      field's plane through one base register); 19 is the FP32 default.
 //  variant 23: variant 17 loading each plane straight into its ring slot (direct-to-LDS loads).
 //  variants 5, 6: the lean z-march with a cell-major LDS ring (FP32; kept for comparison).
-//  The FP64 default also has a build that reads the periodic images of the ghost cells (WRAP),
-//  used by the whole-step entry points of a periodic box.
+//  The defaults also have builds that read the periodic images of the ghost cells (WRAP), used by
+//  the whole-step entry points of a periodic box where they are the faster form.
 #include "physics.h"
 
 namespace ac {
@@ -1679,15 +1679,27 @@ kernel_config_supported(const AcKernelConfig& c)
 bool
 kernel_config_reads_images(const AcKernelConfig& c)
 {
-    // The FP64 default, compiled a second time with WRAP: 3 fills of 0.3 ms saved per 512^3 step
-    // against a slightly slower solve, 29.8 -> 29.2 ms per step. In FP32 the default's WRAP build
-    // measured 4.37 ms per 512^3 launch against 4.08 (the same instruction mix in its plane loop),
-    // more than the 0.15 ms fill it saves, so FP32 keeps the fills.
+    // the defaults (FP32 variant 19, FP64 variant 8), compiled a second time with WRAP
 #if AC_DOUBLE_PRECISION == 1
     return c.variant == 8 && c.tile_x == 16 && c.tile_y == 16 && c.z_depth >= 0;
 #else
-    (void)c;
-    return false;
+    return c.variant == 19 && c.tile_x == 16 && c.tile_y == 16 && c.z_depth >= 0;
+#endif
+}
+
+bool
+default_reads_images(int nx, int ny, int nz)
+{
+    // FP64: the WRAP build saves three 0.3 ms fills per 512^3 step for a solve about 1 % slower
+    // (29.8 -> 29.2 ms per step). FP32: its WRAP build is slower per launch at 512^3 (4.37 against
+    // 4.08 ms; its box-edge tiles fetch extra halo lines), more than the fills it saves (12.69 ->
+    // 13.20 ms per step), but faster per step at 256^3 (1.87 -> 1.74 ms) and 128^3 (0.352 ->
+    // 0.319 ms): images below 384^3 cells.
+#if AC_DOUBLE_PRECISION == 1
+    (void)nx, (void)ny, (void)nz;
+    return true;
+#else
+    return (double)nx * ny * nz < 384.0 * 384.0 * 384.0;
 #endif
 }
 
@@ -1753,6 +1765,9 @@ launch_substep(const AcKernelConfig& c, const SubstepArgs& a, hipStream_t stream
             return hipErrorInvalidConfiguration;
 #if AC_DOUBLE_PRECISION == 1
         return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 1, false, 4, true, true, 7, false, true>,
+                                          a, c.z_depth, stream);
+#else
+        return launch_zmarch_grid<16, 16>(solve_zlean<16, 16, 2, false, 4, true, true, 7, true, true>,
                                           a, c.z_depth, stream);
 #endif
     }

@@ -29,11 +29,13 @@ AcResult acDeviceReduceVecScalDouble(const Device device, const Stream stream,
 /* ---- Whole RK3 step ----
  * The interior result of three times {periodic fill of `in` over the whole m^3 box, substep over
  * the interior, swap}, the caller sequence of the reference's single-GPU node loop
- * (src/core/node.cc:729-806). In the FP64 library with its default configuration (variant 8,
- * 16 x 16 tiles), compensated integration off and n >= 3 on every axis, the substeps read the
- * periodic images of the ghost cells directly and no fill is made; otherwise `in` is filled
- * before each substep. The state ends in `in` without a final fill, as in the reference: its
- * ghost cells are to be filled before they are read. */
+ * (src/core/node.cc:729-806). With the default kernel configuration (FP32 variant 19, FP64
+ * variant 8; 16 x 16 tiles), compensated integration off and n >= 3 on every axis, the substeps
+ * can instead read the periodic images of the ghost cells directly and make no fill: always in
+ * FP64, in FP32 below 384^3 cells (above, its image-reading build is slower than the fills it
+ * saves); acDeviceAutoOptimize measures both forms and keeps the faster, and
+ * acDeviceSetReadPeriodicImages overrides the choice. The state ends in `in` without a final
+ * fill, as in the reference: its ghost cells are to be filled before they are read. */
 AcResult acDeviceIntegrateStep(const Device device, const Stream stream, const AcReal dt);
 
 /* One substep of acDeviceIntegrateStep: over the whole interior of a periodic box, reading the
@@ -42,6 +44,9 @@ AcResult acDeviceIntegrateStep(const Device device, const Stream stream, const A
 AcResult acDeviceIntegrateSubstepPeriodic(const Device device, const Stream stream,
                                           const int step_number, const AcReal dt);
 int acDeviceSubstepReadsPeriodicImages(const Device device);
+/* Whether periodic steps read the ghost cells' images (1) or fill them (0); AC_FAILURE when 1 is
+ * asked of a configuration without an image-reading build. */
+AcResult acDeviceSetReadPeriodicImages(const Device device, const int enable);
 
 /* The step with general boundaries (acNodeIntegrateGBC, node.cc:808-880, on one Device):
  * 3 x {acDeviceGeneralBoundconds over the whole box with the face types of `config`, substep,

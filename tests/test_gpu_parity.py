@@ -386,13 +386,14 @@ def test_beltrami_decay(lib):
 
 @pytest.mark.parametrize("zd", [0, 7])
 def test_step_reads_periodic_images(lib, zd):
-    """acDeviceIntegrateStep in the FP64 library with its default configuration makes no fill:
-    its substeps read the periodic images of the ghost cells (the FP32 default and variant 2 fill
-    before each substep). In each case the step's interior equals, bit for bit, that of the fill
-    sequence made by hand (3 x {acDevicePeriodicBoundconds, acDeviceIntegrateSubstep, swap}),
-    with every ghost cell of both buffer sets NaN (a ghost read would carry a NaN into the
-    interior); the same for the recorded step, for the general-boundary step with periodic faces
-    and for three acDeviceIntegrateSubstepPeriodic calls."""
+    """acDeviceIntegrateStep with the default configuration on a small box makes no fill: its
+    substeps read the periodic images of the ghost cells (in both precisions below 384^3 cells).
+    With the images switched off (acDeviceSetReadPeriodicImages 0), and with variant 2 (no such
+    build), it fills before each substep. In each case the step's interior equals, bit for bit,
+    that of the fill sequence made by hand (3 x {acDevicePeriodicBoundconds,
+    acDeviceIntegrateSubstep, swap}), with every ghost cell of both buffer sets NaN (a ghost read
+    would carry a NaN into the interior); the same for the recorded step, for the general-boundary
+    step with periodic faces and for three acDeviceIntegrateSubstepPeriodic calls."""
     nx, ny, nz = 21, 18, 15
     dt = 1e-3
     info = lib.config(n=nx, ny=ny, nz=nz)
@@ -403,9 +404,11 @@ def test_step_reads_periodic_images(lib, zd):
         g[3:-3, 3:-3, 3:-3] = f[3:-3, 3:-3, 3:-3]
         nan_ghosts.append(g)
 
-    def loaded(config):
+    def loaded(config, images=None):
         dev = make_device(lib, info)
         dev.set_kernel_config(*config)
+        if images is not None:
+            assert lib.dll.acDeviceSetReadPeriodicImages(dev.handle, images) == abi.AC_SUCCESS
         mesh = host_mesh_from(lib, info, nan_ghosts)
         dev.load_mesh(mesh)  # both buffer sets: in, then out through a swap
         dev.swap_buffers()
@@ -414,8 +417,9 @@ def test_step_reads_periodic_images(lib, zd):
         return dev, mesh
 
     lean = (8 if lib.double else 19, 16, 16, zd)
-    for config, modes in ((lean, ("eager", "graph", "gbc", "substeps")),
-                          ((2, 32, 4, zd), ("eager", "substeps"))):
+    for config, modes, images in ((lean, ("eager", "graph", "gbc", "substeps"), None),
+                                  (lean, ("eager", "substeps"), 0),
+                                  ((2, 32, 4, zd), ("eager", "substeps"), None)):
         dev, mesh = loaded(config)
         with dev:
             for s in range(3):
@@ -426,13 +430,14 @@ def test_step_reads_periodic_images(lib, zd):
             dev.store_mesh(ref)
         assert all(np.isfinite(ref.interior(q)).all() for q in range(8))
         for mode in modes:
-            dev, mesh = loaded(config)
+            dev, mesh = loaded(config, images)
             with dev:
                 if mode == "gbc":
                     dev.integrate_step_gbc(info, lib.real(dt))
                 elif mode == "substeps":  # acDeviceIntegrateSubstepPeriodic, one substep at a time
                     reads = lib.dll.acDeviceSubstepReadsPeriodicImages(dev.handle)
-                    assert reads == (1 if (lib.double and config[0] == 8) else 0)
+                    # the default at this size reads images; variant 2 has no such build
+                    assert reads == (0 if config[0] == 2 or images == 0 else 1)
                     for s in range(3):
                         assert lib.dll.acDeviceIntegrateSubstepPeriodic(
                             dev.handle, abi.STREAM_DEFAULT, s, lib.real(dt)) == abi.AC_SUCCESS
