@@ -629,8 +629,15 @@ acEnsembleIntegrateStepLooped(Ensemble e, const AcReal dt)
                 cin[q]  = e->fields[par][q][b];
                 cout[q] = e->fields[1 - par][q][b];
             }
+            // each box as a single-box Device steps it: its default configuration, reading the
+            // ghost cells' periodic images where that is the default form, else fill + launch
+            const AcKernelConfig cfg = default_kernel_config(n.x, n.y, n.z);
+            const bool images        = kernel_config_reads_images(cfg) &&
+                                default_reads_images(n.x, n.y, n.z) && n.x >= NGHOST &&
+                                n.y >= NGHOST && n.z >= NGHOST;
             const int3 zero = make_int3(0, 0, 0), m = make_int3(n.x + 6, n.y + 6, n.z + 6);
-            AC_HIP_CHECK(launch_periodic_fill(cin, NF, n.x, n.y, n.z, zero, m, e->stream));
+            if (!images)
+                AC_HIP_CHECK(launch_periodic_fill(cin, NF, n.x, n.y, n.z, zero, m, e->stream));
             SubstepArgs a;
             for (int q = 0; q < NF; ++q) {
                 a.in[q]   = cin[q];
@@ -644,10 +651,10 @@ acEnsembleIntegrateStepLooped(Ensemble e, const AcReal dt)
             a.mxy   = a.mx * a.my;
             a.start = make_int3(NGHOST, NGHOST, NGHOST);
             a.end   = make_int3(n.x + NGHOST, n.y + NGHOST, n.z + NGHOST);
-            a.wrap  = 0;
+            a.wrap  = images;
             a.phys  = make_phys_params(e->info[b]);
             a.rk    = make_rk3_coeffs(s, dt);
-            AC_HIP_CHECK(launch_substep(default_kernel_config(n.x, n.y, n.z), a, e->stream));
+            AC_HIP_CHECK(launch_substep(cfg, a, e->stream));
         }
         par = 1 - par;
     }

@@ -184,8 +184,10 @@ AcResult acEnsemblePeriodicBoundconds(Ensemble ensemble);
 /* One RK3 step of every box: eager launches, or one HIP graph per (dt, buffer parity) replayed. */
 AcResult acEnsembleIntegrateStep(Ensemble ensemble, const AcReal dt);
 AcResult acEnsembleIntegrateStepGraph(Ensemble ensemble, const AcReal dt);
-/* The same step as a loop of single-box launches (fill + solve per box and substep, each box with
- * its single-box kernel configuration): the baseline the batched step is measured against. */
+/* The same step as a loop of single-box launches, each box stepped as a single-box Device steps
+ * it (its default kernel configuration; a fill + solve per substep, or the solve reading the ghost
+ * cells' periodic images where that is the default form): the baseline the batched step is
+ * measured against. */
 AcResult acEnsembleIntegrateStepLooped(Ensemble ensemble, const AcReal dt);
 /* Diagnostics of the last step, 7 per box: max |u|, rms |u|, max |A|, rms |A|, max lnrho,
  * min lnrho, rms lnrho (FP64, over the interior; synchronous). */
