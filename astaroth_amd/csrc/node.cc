@@ -18,7 +18,9 @@
 //     interior to rounding: the z-march accumulates a cell's cross-derivative sums in the phase
 //     of its unrolled plane loop that the column's first plane sets, and the phases are compiled
 //     separately, so a few cells differ in their last bits. Boxes with an axis shorter than
-//     2 NGHOST integrate the interior in one launch.
+//     2 NGHOST integrate the interior in one launch. Where the Device reads the ghost cells'
+//     periodic images instead of filling them (acDeviceSubstepReadsPeriodicImages), there is no
+//     fill to overlap and acNodeIntegrate runs each substep as one such launch.
 //   acNodeReduce*: the Device reduction, finalised as node.cc:932-957 finalises the per-GPU
 //     results: RMS, RMS_EXP and ALFVEN_RMS return sqrt(sum / (n_x n_y n_z)).
 //   acNodePrintInfo, acNodeAutoOptimize: the Device's (the reference returns AC_FAILURE for both,
@@ -357,6 +359,17 @@ acNodeIntegrate(const Node node, const AcReal dt)
 {
     if (!valid(node, "acNodeIntegrate"))
         return AC_FAILURE;
+    if (acDeviceSubstepReadsPeriodicImages(node->device)) {
+        // the Device's build reads the ghost cells' periodic images: no fill to wait for or to
+        // overlap, so each substep is one launch over the interior
+        if (acDeviceSynchronizeStream(node->device, STREAM_ALL) != AC_SUCCESS)
+            return AC_FAILURE;
+        for (int s = 0; s < 3; ++s)
+            if (acDeviceIntegrateSubstepPeriodic(node->device, STREAM_0, s, dt) != AC_SUCCESS ||
+                acDeviceSwapBuffers(node->device) != AC_SUCCESS)
+                return AC_FAILURE;
+        return acDeviceSynchronizeStream(node->device, STREAM_ALL);
+    }
     const int3 m = node->grid.m;
     return split_step(node, dt, [&](Stream s) {
         return acDevicePeriodicBoundconds(node->device, s, i3(0, 0, 0), m);

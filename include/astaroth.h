@@ -351,7 +351,9 @@ Node acGetNode(void);
  *   integrated on STREAM_16 while the ghost shell is filled (periodic / by the face types of
  *   `config`) on STREAM_0, then the six outer slabs on STREAM_0..5, then the buffers swap; the
  *   result is that of 3 x {fill, substep over the interior, swap} to rounding (the launches
- *   start their z-march columns at other planes); synchronous;
+ *   start their z-march columns at other planes); synchronous. Where the Device's periodic step
+ *   reads the ghost cells' images (acDeviceSubstepReadsPeriodicImages), acNodeIntegrate is that
+ *   step instead;
  * - acNodeReduce* finalise RMS, RMS_EXP and ALFVEN_RMS as sqrt(sum / n_x n_y n_z);
  * - acNodePrintInfo and acNodeAutoOptimize are the Device's (the reference returns AC_FAILURE).
  * The load, store, set and reduction calls first synchronise their stream, as the reference

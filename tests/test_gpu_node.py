@@ -1,13 +1,15 @@
 """The Node interface (reference include/astaroth.h:355-500, src/core/node.cc) on one GPU,
 through the C ABI.
 
-* acNodeIntegrate runs each substep as the inner region [2 NGHOST, n) on STREAM_16 while the
-  ghost shell is filled on STREAM_0, then the six outer slabs on STREAM_0..5; its interior equals
+* acNodeIntegrate (with its Device filling ghost cells) runs each substep as the inner region
+  [2 NGHOST, n) on STREAM_16 while the ghost shell is filled on STREAM_0, then the six outer slabs
+  on STREAM_0..5; its interior equals
   that of acDeviceIntegrateStep (one launch over the interior per substep) to rounding (range-
   normalised 2e-6 in FP32, 1e-13 in FP64, the bar between kernel configurations: a cell's
   cross-derivative sums are accumulated in the plane-loop phase that its column's start sets,
   and the phases are compiled separately, so a few cells differ in their last bits), and bit for
-  bit on a box with an axis shorter than 2 NGHOST, where the Node integrates in one launch.
+  bit on a box with an axis shorter than 2 NGHOST, where the Node integrates in one launch, and
+  where the Device reads the ghost cells' periodic images (one launch per substep, no fill).
   acNodeIntegrateGBC with symmetric / antisymmetric faces equals acDeviceIntegrateStepGBC to the
   same bar, ghost cells included.
 * acNodeIntegrateSubstep clips its region to the interior.
@@ -50,11 +52,15 @@ def host_mesh(lib, info, seed_config=1):
     return HostMesh(lib, info, mhd_fields(nx, ny, nz, np.float64, config_id=seed_config))
 
 
-@pytest.mark.parametrize("dims", [(24, 20, 18), (24, 5, 18)], ids=["split", "thin-y"])
-def test_node_integrate_equals_device_step(lib, dims):
+@pytest.mark.parametrize("dims,images", [((24, 20, 18), 0), ((24, 20, 18), 1), ((24, 5, 18), 0)],
+                         ids=["split", "images", "thin-y"])
+def test_node_integrate_equals_device_step(lib, dims, images):
     info = lib.config(n=dims[0], ny=dims[1], nz=dims[2])
     mesh = host_mesh(lib, info)
     with lib.node_create(info, autotune=False) as node:
+        # the Node's Device: fills (the inner/outer split) or reads the ghost cells' images
+        dev0 = node.query_device_configuration().devices[0]
+        assert lib.dll.acDeviceSetReadPeriodicImages(ctypes.c_void_p(dev0), images) == abi.AC_SUCCESS
         node.load_mesh(mesh)
         for _ in range(2):
             node.integrate(1e-3)
@@ -64,7 +70,7 @@ def test_node_integrate_equals_device_step(lib, dims):
         for _ in range(2):
             dev.integrate_step(1e-3)
         ref = dev.store_mesh(mesh.copy())
-    if dims[1] < 6:  # one launch over the interior: the same kernel arithmetic, cell for cell
+    if dims[1] < 6 or images:  # one launch over the interior: the same arithmetic, cell for cell
         for q in range(8):
             assert np.array_equal(got.interior(q), ref.interior(q)), abi.VTXBUF_NAMES[q]
     else:
