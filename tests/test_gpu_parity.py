@@ -451,6 +451,30 @@ def test_step_reads_periodic_images(lib, zd):
                 assert np.array_equal(out.interior(q), ref.interior(q)), (mode, config, FIELD_NAMES[q])
 
 
+def test_periodic_image_switch(lib):
+    """acDeviceSetReadPeriodicImages is refused (AC_FAILURE) for a configuration without an
+    image-reading build, and compensated integration turns image reads off. By default a small box
+    reads images and a 512^3 FP32 box does not (the measured crossover lies between 256^3 and
+    512^3 in FP32; FP64 reads them at every size)."""
+    info = lib.config(n=24)
+    with make_device(lib, info) as dev:
+        assert lib.dll.acDeviceSubstepReadsPeriodicImages(dev.handle) == 1
+        dev.set_kernel_config(2, 32, 4, 0)
+        assert lib.dll.acDeviceSetReadPeriodicImages(dev.handle, 1) == abi.AC_FAILURE
+        assert lib.dll.acDeviceSubstepReadsPeriodicImages(dev.handle) == 0
+        dev.set_kernel_config(8 if lib.double else 19, 16, 16, 0)
+        assert lib.dll.acDeviceSetReadPeriodicImages(dev.handle, 1) == abi.AC_SUCCESS
+        assert lib.dll.acDeviceSubstepReadsPeriodicImages(dev.handle) == 1
+        dev.set_compensated_integration(True)
+        assert lib.dll.acDeviceSubstepReadsPeriodicImages(dev.handle) == 0
+        dev.set_compensated_integration(False)
+        assert lib.dll.acDeviceSetReadPeriodicImages(dev.handle, 0) == abi.AC_SUCCESS
+        assert lib.dll.acDeviceSubstepReadsPeriodicImages(dev.handle) == 0
+    big = lib.config(n=512)
+    with make_device(lib, big) as dev:
+        assert lib.dll.acDeviceSubstepReadsPeriodicImages(dev.handle) == (1 if lib.double else 0)
+
+
 def test_graph_step_matches_eager(lib):
     """acDeviceIntegrateStepGraph replays give the same state as eager steps, replay after
     replay, with eager work and synchronisation in between."""
