@@ -109,6 +109,17 @@ split_step(const Node node, const AcReal dt, Fill fill)
                 return AC_FAILURE;
             if (acDeviceSynchronizeStream(d, STREAM_0) != AC_SUCCESS) // the ghost shell is filled
                 return AC_FAILURE;
+            // The slabs are thin: z-march columns 14 planes deep give their launches enough
+            // blocks to spread over the GPU (a whole-depth column per 16 x 16 tile leaves a side
+            // slab of a 512^3 box 32 blocks). The inner launch has already taken its own depth.
+            AcKernelConfig whole;
+            if (acDeviceGetKernelConfig(d, &whole) != AC_SUCCESS)
+                return AC_FAILURE;
+            AcKernelConfig thin = whole;
+            if (thin.variant != 0)
+                thin.z_depth = 14;
+            if (acDeviceSetKernelConfig(d, thin) != AC_SUCCESS)
+                return AC_FAILURE;
             const struct {
                 Stream stream;
                 int3 start, end;
@@ -120,9 +131,12 @@ split_step(const Node node, const AcReal dt, Fill fill)
                 {STREAM_4, i3(N, 2 * N, 2 * N), i3(2 * N, n.y, n.z)},            // left
                 {STREAM_5, i3(n.x, 2 * N, 2 * N), i3(n.x + N, n.y, n.z)},        // right
             };
+            AcResult r = AC_SUCCESS;
             for (const auto& sl : slabs)
-                if (acDeviceIntegrateSubstep(d, sl.stream, s, sl.start, sl.end, dt) != AC_SUCCESS)
-                    return AC_FAILURE;
+                if (r == AC_SUCCESS)
+                    r = acDeviceIntegrateSubstep(d, sl.stream, s, sl.start, sl.end, dt);
+            if (acDeviceSetKernelConfig(d, whole) != AC_SUCCESS || r != AC_SUCCESS)
+                return AC_FAILURE;
         }
         if (acDeviceSynchronizeStream(d, STREAM_ALL) != AC_SUCCESS ||
             acDeviceSwapBuffers(d) != AC_SUCCESS)
