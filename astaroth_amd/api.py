@@ -5,7 +5,7 @@ the reference's names and argument meaning (src/core/device.cc, include/astaroth
 test reads like the reference's own samples:
 
     lib  = Library(double=True)
-    info = lib.config(n=64)                    # AcMeshInfo of config 1
+    info = lib.config(n=64)                    # AcMeshInfo of a 64^3 box
     mesh = HostMesh(lib, info)                 # 8 numpy arrays of m^3 cells
     dev  = lib.device_create(0, info)          # acDeviceCreate
     dev.load_mesh(mesh); dev.integrate_step(dt); dev.store_mesh(mesh)

@@ -5,8 +5,8 @@ acDeviceRandomizeUniform evaluates on the GPU (csrc/init.hip); `uniform_interior
 bit for bit on the host so small cases can be generated on either side. Seeds are derived from
 (config id, box id, field id) and are recorded with every result.
 
-Config 1-3 distributions: lnrho ~ U(-1e-2, 1e-2); u and A components ~ U(-1e-3, 1e-3);
-entropy = 0 (config 3 adds u_x += 1e-2 sin(2 z)).
+The uniform-field distribution: lnrho ~ U(-1e-2, 1e-2); u and A components ~ U(-1e-3, 1e-3);
+entropy = 0 (the perturbed state adds u_x += 1e-2 sin(2 z)).
 """
 from __future__ import annotations
 
@@ -53,7 +53,7 @@ def periodic_fill(f: np.ndarray) -> np.ndarray:
 
 
 def mhd_fields(nx, ny, nz, dtype=np.float64, config_id=1, box_id=0, perturb_sin2z=False):
-    """Eight (mz, my, mx) arrays of the config-1 distribution, ghosts filled periodically."""
+    """Eight (mz, my, mx) arrays of the uniform-field distribution, ghosts filled periodically."""
     out = []
     for name, (lo, hi) in FIELD_RANGES.items():
         q = VTX[name]

@@ -4,7 +4,7 @@
 One step = 3 x {periodic halo fill, fused solve + RK3 substep, buffer swap} over the whole box,
 driven through the C ABI (acDevicePeriodicBoundconds, acDeviceIntegrateSubstep,
 acDeviceSwapBuffers), the caller sequence of the reference's single-GPU step
-(src/core/node.cc:729-806). Inputs are generated on the GPU (seeded splitmix64, the config-1
+(src/core/node.cc:729-806). Inputs are generated on the GPU (seeded splitmix64, the uniform-field
 distribution) and stay resident in HBM; the timed region is K steps bracketed by device
 synchronisation. `value` is the FP32 rate; the FP64 build is timed on the same workload.
 
@@ -178,7 +178,7 @@ def step_diagnostics(dev):
 
 
 def precision_study(n: int, steps: int, dt: float, hip: Hip):
-    """512^3 precision study: the same perturbed config-1 state (u_x += 1e-2 sin 2z) advanced
+    """512^3 precision study: the same perturbed uniform-field state (u_x += 1e-2 sin 2z) advanced
     `steps` RK3 steps by the FP32 and the FP64 library; per-field error of FP32 against FP64 over
     the interior, range-normalised and at the point of maximum absolute error."""
     import numpy as np
@@ -293,7 +293,7 @@ def cpu_baseline(n=128, steps=3):
     pyoracle.integrate_steps(fields, rp, 1e-3, steps)
     dt = time.perf_counter() - t0
     return dict(value=n**3 * steps / dt, unit="cell-updates/s", cores=threads, kind="port",
-                sample="%d^3 config-1 box, %d RK3 step(s), FP64 host-form model solver, %.2f s" % (n, steps, dt))
+                sample="%d^3 uniform-field box, %d RK3 step(s), FP64 host-form model solver, %.2f s" % (n, steps, dt))
 
 
 def main(argv=None):
@@ -313,11 +313,11 @@ def main(argv=None):
                     help="force a solve configuration 'variant,tile_x,tile_y,z_depth' instead of autotuning")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="REPS",
                     help="only run REPS launches of the plain copy kernel and print its bandwidth")
-    ap.add_argument("--workload", choices=["headline", "config2", "config3", "config4", "config5"],
+    ap.add_argument("--workload", choices=["headline", "sweep256", "precision512", "ensemble96", "halo1024"],
                     default="headline",
-                    help="headline: the 512^3 FP32 vs FP64 metric (default). config2: 256^3 with per-step "
-                         "FP64 diagnostics and the tuning sweep. config3: 512^3 FP32-vs-FP64 precision "
-                         "study. config4: 96-box FP32 ensemble, batched vs looped. config5: 1024^3 FP32 with "
+                    help="headline: the 512^3 FP32 vs FP64 metric (default). sweep256: 256^3 with per-step "
+                         "FP64 diagnostics and the tuning sweep. precision512: 512^3 FP32-vs-FP64 precision "
+                         "study. ensemble96: 96-box FP32 ensemble, batched vs looped. halo1024: 1024^3 FP32 with "
                          "the 26-block pack/unpack halo")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -355,7 +355,7 @@ def main(argv=None):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if main_prec == "fp32" else "f64",
-        "data": "synthetic: seeded splitmix64 config-1 fields (lnrho~U(-1e-2,1e-2), u,A~U(-1e-3,1e-3), s=0)",
+        "data": "synthetic: seeded splitmix64 uniform fields (lnrho~U(-1e-2,1e-2), u,A~U(-1e-3,1e-3), s=0)",
         "config": {
             "workload": "8-field compressible MHD, %d^3 periodic box, 6th-order stencils, 2N-RK3, dt=%g" % (a.n, a.dt),
             "grid": [a.n, a.n, a.n],
@@ -397,20 +397,20 @@ def main(argv=None):
 def run_workload(a, kcfg, hip):
     """The other measured configurations; each prints one JSON line of its own."""
     base = {"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "data": "synthetic: seeded splitmix64 fields (config-1 distribution)"}
-    if a.workload == "config3":
+            "data": "synthetic: seeded splitmix64 uniform fields"}
+    if a.workload == "precision512":
         n = a.n
         steps = 10
         r = precision_study(n, steps, a.dt, hip)
         line = dict(base, metric="FP32 vs FP64 range-normalised max error after %d RK3 steps at %d^3" % (steps, n),
                     value=r["worst_range_normalised"], unit="relative", higher_is_better=False, steps=steps,
                     warmup=0, dtype="f32 vs f64",
-                    config={"workload": "precision study: config-1 state + u_x 1e-2 sin(2z), %d^3, dt=%g" % (n, a.dt)},
+                    config={"workload": "precision study: uniform-field state + u_x 1e-2 sin(2z), %d^3, dt=%g" % (n, a.dt)},
                     precision=r)
-        line["data"] = "synthetic: seeded splitmix64 config-3 fields (config-1 + u_x 1e-2 sin 2z)"
+        line["data"] = "synthetic: seeded splitmix64 perturbed fields (uniform fields + u_x 1e-2 sin 2z)"
         print(json.dumps(line), flush=True)
         return 0
-    if a.workload == "config5":
+    if a.workload == "halo1024":
         n = 1024 if a.n == 512 else a.n
         # The explicit grad-div (bulk + shear) viscosity limits dt ~ dx^2: dt = 1e-3 is inside the
         # RK3 stability region at 512^3 (nu_eff dt |lambda_max| = 2.0 < 2.5) but not at 1024^3, so
@@ -437,7 +437,7 @@ def run_workload(a, kcfg, hip):
                     diagnostics_last_step=r["reductions"][-1] if r["reductions"] else None)
         print(json.dumps(line), flush=True)
         return 0
-    if a.workload == "config4":
+    if a.workload == "ensemble96":
         r = ensemble_benchmark(a.dt, hip)
         best = r["batched_graph"]
         line = dict(base, metric="box-cell-updates/s per RK3 step, 96-box FP32 ensemble (one graph per step)",
@@ -450,7 +450,7 @@ def run_workload(a, kcfg, hip):
         line["vs_looped"] = best["value"] / r["looped"]["value"]
         print(json.dumps(line), flush=True)
         return 0
-    if a.workload == "config2":
+    if a.workload == "sweep256":
         n = 256 if a.n == 512 else a.n
         out = {}
         for prec in ("fp32", "fp64"):

@@ -193,7 +193,7 @@ def test_wide_box_strip_order(lib):
 
 
 def test_config1_fp64_parity_with_oracle():
-    """Config 1: 64^3, dt = 1e-3, 5 RK3 steps, FP64, acVerifyMesh criterion per field.
+    """The 64^3 uniform-field box: dt = 1e-3, 5 RK3 steps, FP64, acVerifyMesh criterion per field.
 
     Gate: acEvalError against the oracle's restatement of the same (two-buffer) RK3 form in FP64.
     Reported: against the host model solver in long double (sanity bound 8 ulp) and in FP64. The
@@ -206,11 +206,11 @@ def test_config1_fp64_parity_with_oracle():
     fields = mhd_fields(n, n, n, np.float64, config_id=1)
     gpu = run_gpu_steps(lib, info, fields, 1e-3, 5)
     same_form = verify_all(oracle_steps(info, fields, 1e-3, 5, gpu_form=True), gpu,
-                           "config-1 FP64 GPU vs oracle, same RK3 form, FP64")
+                           "64^3 FP64 GPU vs oracle, same RK3 form, FP64")
     exact = verify_all(oracle_steps(info, fields, 1e-3, 5, dtype=np.longdouble), gpu,
-                       "config-1 FP64 GPU vs host model solver in long double")
+                       "64^3 FP64 GPU vs host model solver in long double")
     verify_all(oracle_steps(info, fields, 1e-3, 5), gpu,
-               "config-1 FP64 GPU vs host model solver in FP64 (reported)")
+               "64^3 FP64 GPU vs host model solver in FP64 (reported)")
     for q in range(8):
         assert same_form[q]["ok"] and same_form[q]["ulp"] <= 3, (FIELD_NAMES[q], same_form[q])
         # the FP64 two-buffer form itself carries a few ulps of rounding against long double
@@ -266,14 +266,14 @@ def test_substep_matches_oracle_substep(lib, step):
 
 
 def test_fp32_error_against_fp64_oracle():
-    """FP32 build vs the FP64 oracle after 5 steps of config 1 (range-normalised, per field)."""
+    """FP32 build vs the FP64 oracle after 5 steps of the 64^3 uniform-field box (range-normalised, per field)."""
     lib = Library(False)
     n = 64
     info = lib.config(n=n)
     fields = mhd_fields(n, n, n, np.float64, config_id=1)
     gpu = run_gpu_steps(lib, info, fields, 1e-3, 5)
     ref = oracle_steps(Library(True).config(n=n), fields, 1e-3, 5)
-    print("\nconfig-1 FP32 GPU vs FP64 oracle (max |err| / max |model|):")
+    print("\n64^3 FP32 GPU vs FP64 oracle (max |err| / max |model|):")
     for q in range(8):
         scale = np.max(np.abs(ref[q]))
         err = np.max(np.abs(gpu[q].astype(np.float64) - ref[q])) / scale

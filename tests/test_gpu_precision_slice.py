@@ -1,6 +1,6 @@
 """The 512^3 precision study against the oracle in long double, on a small slice.
 
-The periodic 512^3 box in the perturbed state (config-1 fields, u_x += 1e-2 sin 2z) is advanced
+The periodic 512^3 box in the perturbed state (uniform fields, u_x += 1e-2 sin 2z) is advanced
 10 RK3 steps at dt = 1e-3 by the FP64 and by the FP32 library (acDeviceIntegrateStep). The oracle
 cannot step the whole box in long double within a test, but a cell after 10 steps depends only on
 the cells within 3 x 30 = 90 of it (one 3-cell stencil reach per substep). So a 4^3 slice at the
