@@ -165,3 +165,27 @@ def test_get_node_views_the_legacy_device(lib):
         assert out.value == lib.dll.acReduceScal(abi.RT["RTYPE_MAX"], abi.VTX["VTXBUF_ENTROPY"])
     finally:
         assert lib.dll.acQuit() == abi.AC_SUCCESS
+
+
+def test_node_load_with_offset_copies_src_to_dst(lib):
+    """acNodeLoadVertexBufferWithOffset copies num_vertices cells from `src` of the host mesh to
+    `dst` of the device, as the Device form does (the reference Node passes `src` for both)."""
+    n = 12
+    info = lib.config(n=n)
+    mesh = host_mesh(lib, info)
+    m = n + 6
+    with lib.node_create(info, autotune=False) as node:
+        zero = HostMesh(lib, info)
+        node.load_mesh(zero)
+        q = abi.VTX["VTXBUF_UUY"]
+        r = lib.dll.acNodeLoadVertexBufferWithOffset(node.handle, abi.STREAM_DEFAULT, mesh.struct, q,
+                                                     int3(0, 0, 2), int3(0, 0, 9), 2 * m * m)
+        assert r == abi.AC_SUCCESS
+        node.synchronize()
+        out = node.store_mesh(HostMesh(lib, info))
+    assert np.array_equal(out.fields[q][9:11], mesh.fields[q][2:4])
+    assert not out.fields[q][:9].any() and not out.fields[q][11:].any()
+    for other in range(8):
+        if other != q:
+            assert not out.fields[other].any()
+
