@@ -453,6 +453,19 @@ acDeviceCreate(const int id, const AcMeshInfo config, Device* device_handle)
         fprintf(stderr, "acDeviceCreate: invalid grid %d x %d x %d\n", nx, ny, nz);
         return AC_FAILURE;
     }
+    // The buffers hold (n + 2 NGHOST) cells per axis, while loads, stores and fills size their
+    // copies from AC_mx/my/mz (acVertexBufferSize, acVertexBufferIdx): a configuration whose
+    // built-in sizes are unset or stale would copy past the allocation or copy nothing.
+    if (config.int_params[AC_mx] != nx + 2 * NGHOST || config.int_params[AC_my] != ny + 2 * NGHOST ||
+        config.int_params[AC_mz] != nz + 2 * NGHOST ||
+        config.int_params[AC_mxy] != config.int_params[AC_mx] * config.int_params[AC_my]) {
+        fprintf(stderr,
+                "acDeviceCreate: AC_mx, AC_my, AC_mz = %d, %d, %d (AC_mxy %d) do not match the grid "
+                "%d x %d x %d with %d ghost cells per side; call acHostUpdateBuiltinParams first\n",
+                config.int_params[AC_mx], config.int_params[AC_my], config.int_params[AC_mz],
+                config.int_params[AC_mxy], nx, ny, nz, NGHOST);
+        return AC_FAILURE;
+    }
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess) {
         (void)hipGetLastError();

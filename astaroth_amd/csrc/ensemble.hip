@@ -360,8 +360,11 @@ acEnsembleCreate(const int device_id, const int nboxes, const AcMeshInfo* config
     }
     for (int b = 0; b < nboxes; ++b) {
         const int3 n = dims_of(configs[b]);
-        if (n.x < 1 || n.y < 1 || n.z < 1) {
-            fprintf(stderr, "acEnsembleCreate: box %d has an empty grid\n", b);
+        // The batched solve reads the periodic images of the ghost cells instead of filled
+        // ghosts; an image lies in the interior only if every axis holds NGHOST cells or more.
+        if (n.x < NGHOST || n.y < NGHOST || n.z < NGHOST) {
+            fprintf(stderr, "acEnsembleCreate: box %d is %d x %d x %d; every axis needs at least %d cells\n",
+                    b, n.x, n.y, n.z, NGHOST);
             return AC_FAILURE;
         }
     }

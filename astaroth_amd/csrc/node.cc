@@ -174,10 +174,13 @@ acNodeCreate(const int id, const AcMeshInfo node_config, Node* node_handle)
     if (!node)
         return AC_FAILURE;
     node->id          = id;
-    node->config      = node_config;
+    // the built-in sizes follow from the grid, as the reference derives its subgrid's
+    // (node.cc:240-242)
+    node->config = node_config;
+    acHostUpdateBuiltinParams(&node->config);
     node->owns_device = true;
     node->grid        = dims_of(node_config);
-    if (acDeviceCreate(0, node_config, &node->device) != AC_SUCCESS) {
+    if (acDeviceCreate(0, node->config, &node->device) != AC_SUCCESS) {
         delete node;
         return AC_FAILURE;
     }

@@ -46,6 +46,56 @@ def uniform_interior(nx, ny, nz, lo, hi, seed) -> np.ndarray:
     return (lo + (hi - lo) * u).reshape(nz, ny, nx)
 
 
+def uniform_cells(nx, ny, nz, lo, hi, seed, i, j, k) -> np.ndarray:
+    """The generator's value at interior cells (i, j, k) (broadcast integer arrays, taken modulo
+    the box: the periodic image of any cell), FP64."""
+    i, j, k = (np.asarray(v, dtype=np.int64) % n for v, n in ((i, nx), (j, ny), (k, nz)))
+    with np.errstate(over="ignore"):
+        t = (i + nx * (j + ny * k)).astype(np.uint64) + np.uint64(1)
+        z = np.uint64(seed) + t * GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * M1
+        z = (z ^ (z >> np.uint64(27))) * M2
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return lo + (hi - lo) * u
+
+
+def mhd_block(nx, ny, nz, origin, width, dtype=np.float64, config_id=1, box_id=0):
+    """Eight arrays of shape (width_z, width_y, width_x) holding the uniform-field state of an
+    nx x ny x nz periodic box on the block of interior cells [origin, origin + width) (any
+    integers: cells outside the interior are their periodic images). Equal, cell for cell, to
+    the same block cut from mhd_fields, without forming the whole box."""
+    (x0, y0, z0), (wx, wy, wz) = origin, width
+    k = np.arange(z0, z0 + wz)[:, None, None]
+    j = np.arange(y0, y0 + wy)[None, :, None]
+    i = np.arange(x0, x0 + wx)[None, None, :]
+    out = []
+    for name, (lo, hi) in FIELD_RANGES.items():
+        q = VTX[name]
+        if hi == lo:
+            f = np.full((wz, wy, wx), lo, dtype=np.float64)
+        else:
+            f = uniform_cells(nx, ny, nz, lo, hi, seed_for(config_id, box_id, q), i, j, k)
+        out.append(f.astype(dtype))
+    return out
+
+
+def ensemble96_boxes(count=96, seed=4):
+    """Box edges and viscosities of the 96-box ensemble: edges drawn from {32, 48, 64, 96, 128},
+    every even box cubic and every odd box anisotropic, nu ~ U[1e-3, 1e-2] per box."""
+    rng = np.random.default_rng(seed)
+    edges = np.array([32, 48, 64, 96, 128])
+    boxes, nus = [], []
+    for b in range(count):
+        if b % 2 == 0:  # cubic
+            e = int(rng.choice(edges))
+            boxes.append((e, e, e))
+        else:  # anisotropic
+            boxes.append(tuple(int(v) for v in rng.choice(edges, 3)))
+        nus.append(float(rng.uniform(1e-3, 1e-2)))
+    return boxes, nus
+
+
 def periodic_fill(f: np.ndarray) -> np.ndarray:
     """Ghost cells of one (mz, my, mx) array from the wrapped interior (in place)."""
     f[...] = np.pad(f[3:-3, 3:-3, 3:-3], 3, mode="wrap")

@@ -468,31 +468,14 @@ def run_workload(a, kcfg, hip):
     raise ValueError(a.workload)
 
 
-def ensemble_boxes(count=96, seed=4):
-    """Box edges and viscosities of the ensemble workload, drawn from `seed`."""
-    import numpy as np
-
-    rng = np.random.default_rng(seed)
-    edges = np.array([32, 48, 64, 96, 128])
-    boxes, nus = [], []
-    for b in range(count):
-        if b % 2 == 0:  # cubic
-            e = int(rng.choice(edges))
-            boxes.append((e, e, e))
-        else:  # anisotropic
-            boxes.append(tuple(int(v) for v in rng.choice(edges, 3)))
-        nus.append(float(rng.uniform(1e-3, 1e-2)))
-    return boxes, nus
-
-
 def ensemble_benchmark(dt, hip, steps=10, warmup=2):
     """The 96-box ensemble advanced by the batched step (eager and graph) and by a loop of
     single-box launches; box-cell-updates/s = sum of interior cells / step time."""
     from astaroth_amd.api import Library
-    from astaroth_amd.initial import ensemble_init_mhd
+    from astaroth_amd.initial import ensemble_init_mhd, ensemble96_boxes
 
     lib = Library(False)
-    boxes, nus = ensemble_boxes()
+    boxes, nus = ensemble96_boxes()
     infos = [lib.config(n=b[0], ny=b[1], nz=b[2], AC_nu_visc=nu) for b, nu in zip(boxes, nus)]
     cells = float(sum(b[0] * b[1] * b[2] for b in boxes))
     out = {"boxes": boxes, "cells": cells, "steps": steps, "warmup": warmup}

@@ -105,3 +105,19 @@ def test_host_mesh_helpers(lib):
     assert lib.dll.acHostMeshDestroy(ctypes.byref(mesh)) == 0
     hm = HostMesh(lib, info)
     assert hm.fields[0].shape == (14, 14, 14)
+
+
+def test_device_create_rejects_stale_builtin_sizes(lib, capfd):
+    """acDeviceCreate refuses a configuration whose AC_mx/my/mz (which size every load, store and
+    fill) disagree with the grid, before it touches a GPU: unset, and stale from another grid."""
+    info = lib.config(n=16, ny=12, nz=8)
+    handle = ctypes.c_void_p()
+    unset = lib.MeshInfo.from_buffer_copy(info)
+    for k in ("AC_mx", "AC_my", "AC_mz", "AC_mxy"):
+        unset.int_params[abi.IP[k]] = 0
+    stale = lib.MeshInfo.from_buffer_copy(info)
+    stale.int_params[abi.IP["AC_nx"]] = 10  # grid changed after the sizes were derived
+    for bad in (unset, stale):
+        assert lib.dll.acDeviceCreate(0, bad, ctypes.byref(handle)) == abi.AC_FAILURE
+        assert not handle.value
+        assert "call acHostUpdateBuiltinParams first" in capfd.readouterr().err

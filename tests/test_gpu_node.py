@@ -52,8 +52,9 @@ def host_mesh(lib, info, seed_config=1):
     return HostMesh(lib, info, mhd_fields(nx, ny, nz, np.float64, config_id=seed_config))
 
 
-@pytest.mark.parametrize("dims,images", [((24, 20, 18), 0), ((24, 20, 18), 1), ((24, 5, 18), 0)],
-                         ids=["split", "images", "thin-y"])
+@pytest.mark.parametrize("dims,images", [((24, 20, 18), 0), ((24, 20, 18), 1), ((24, 5, 18), 0),
+                                         ((24, 20, 40), 0)],
+                         ids=["split", "images", "thin-y", "multi-column-slabs"])
 def test_node_integrate_equals_device_step(lib, dims, images):
     info = lib.config(n=dims[0], ny=dims[1], nz=dims[2])
     mesh = host_mesh(lib, info)

@@ -217,6 +217,37 @@ def test_config1_fp64_parity_with_oracle():
         assert exact[q]["ulp"] <= 8, (FIELD_NAMES[q], exact[q])
 
 
+@pytest.mark.parametrize("dt,steps", [(float(np.finfo(np.float32).eps), 1), (1e-3, 1), (1e-3, 5)],
+                         ids=["flt-eps-1-step", "1e-3-1-step", "1e-3-5-steps"])
+def test_fp64_against_host_model_solver(dt, steps):
+    """The FP64 library against the host model solver itself (acHostIntegrateStep's form: w stored
+    explicitly, modelsolver.c:987-1041), in FP64, with the reference's acEvalError on every field.
+
+    dt = FLT_EPSILON, 1 step is the reference's own GPU-vs-host check (samples/mpitest/main.cc:
+    343-363); dt = 1e-3, 1 step is the run's time step. Both are gated for every field.
+    After 5 steps at dt = 1e-3 the two RK3 forms part by up to 5 ulps at the max-abs point on
+    the CPU alone (the oracle's FP64 host form against its FP64 two-buffer form: lnrho and u_y
+    at exactly 5 ulps), while the FP64 host form is itself 4-5 ulps from the long-double
+    solution. There every field must pass acEvalError, or else lie within 5 ulps of the FP64 host
+    solver and no farther from the long-double host solution than the FP64 host solver is."""
+    lib = Library(True)
+    n = 64
+    info = lib.config(n=n)
+    fields = mhd_fields(n, n, n, np.float64, config_id=1)
+    gpu = run_gpu_steps(lib, info, fields, dt, steps)
+    host = oracle_steps(info, fields, dt, steps)
+    exact = oracle_steps(info, fields, dt, steps, dtype=np.longdouble)
+    res = verify_all(host, gpu, "64^3 FP64 GPU vs host model solver in FP64, dt %g, %d step(s)" % (dt, steps))
+    to_exact = verify_all(exact, gpu, "  GPU vs host model solver in long double")
+    host_to_exact = verify_all(exact, host, "  FP64 host model solver vs long double")
+    for q in range(8):
+        if steps == 1:
+            assert res[q]["ok"], (FIELD_NAMES[q], res[q])
+        else:
+            assert res[q]["ok"] or (res[q]["ulp"] <= 5 and to_exact[q]["abs"] <= host_to_exact[q]["abs"]), \
+                (FIELD_NAMES[q], res[q], to_exact[q], host_to_exact[q])
+
+
 @pytest.mark.parametrize("dims", [(40, 24, 12), (13, 70, 9)])
 def test_odd_grid_fp64_parity_with_oracle(dims):
     """Non-power-of-two, anisotropic grids with partial tiles: FP64 parity per field."""
