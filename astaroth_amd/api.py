@@ -168,6 +168,8 @@ class Library:
             "acEnsembleBoxCount": (i, [vp]),
             "acEnsembleSetKernelConfig": (i, [vp, abi.KernelConfig]),
             "acEnsembleGetKernelConfig": (i, [vp, ctypes.POINTER(abi.KernelConfig)]),
+            "acEnsembleAutoOptimize": (i, [vp]),
+            "acEnsembleBoxZDepth": (i, [vp, i]),
             "acEnsembleRandomizeUniform": (i, [vp, i, i, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong]),
             "acEnsembleSetVertexBuffer": (i, [vp, i, i, R]),
             "acEnsembleLoadMesh": (i, [vp, i, M]),
@@ -274,8 +276,8 @@ class Library:
     def node_create(self, info, node_id: int = 0, autotune: bool | None = None) -> "Node":
         return Node(self, info, node_id, autotune)
 
-    def ensemble_create(self, infos, device_id: int = 0) -> "Ensemble":
-        return Ensemble(self, infos, device_id)
+    def ensemble_create(self, infos, device_id: int = 0, autotune: bool | None = None) -> "Ensemble":
+        return Ensemble(self, infos, device_id, autotune)
 
     def kernel_configs(self):
         out = []
@@ -349,6 +351,15 @@ class Device:
                     os.environ["AC_AUTOTUNE"] = old
         if res != abi.AC_SUCCESS:
             raise AstarothError("acDeviceCreate failed")
+
+    @classmethod
+    def view(cls, lib: Library, info, handle) -> "Device":
+        """A Device object over an existing handle (a Node's Device, say), which it does not own:
+        do not destroy it."""
+        self = cls.__new__(cls)
+        self.lib, self.info, self.dims = lib, info, dims_of(info)
+        self.handle = ctypes.c_void_p(handle)
+        return self
 
     # context manager
     def __enter__(self):
@@ -572,12 +583,23 @@ class Ensemble:
 
     DIAGNOSTICS = ("max_u", "rms_u", "max_a", "rms_a", "max_lnrho", "min_lnrho", "rms_lnrho")
 
-    def __init__(self, lib: Library, infos, device_id: int = 0):
+    def __init__(self, lib: Library, infos, device_id: int = 0, autotune=None):
         self.lib = lib
         self.infos = list(infos)
         arr = (abi.structs(lib.double)[0] * len(self.infos))(*self.infos)
         self.handle = ctypes.c_void_p()
-        if lib.dll.acEnsembleCreate(device_id, len(self.infos), arr, ctypes.byref(self.handle)) != abi.AC_SUCCESS:
+        old = os.environ.get("AC_AUTOTUNE")
+        if autotune is not None:
+            os.environ["AC_AUTOTUNE"] = "1" if autotune else "0"
+        try:
+            res = lib.dll.acEnsembleCreate(device_id, len(self.infos), arr, ctypes.byref(self.handle))
+        finally:
+            if autotune is not None:
+                if old is None:
+                    del os.environ["AC_AUTOTUNE"]
+                else:
+                    os.environ["AC_AUTOTUNE"] = old
+        if res != abi.AC_SUCCESS:
             raise AstarothError("acEnsembleCreate failed")
 
     def __enter__(self):
@@ -608,6 +630,13 @@ class Ensemble:
         c = abi.KernelConfig()
         self._check(self.lib.dll.acEnsembleGetKernelConfig(self.handle, ctypes.byref(c)), "acEnsembleGetKernelConfig")
         return c
+
+    def autotune(self):
+        return self._check(self.lib.dll.acEnsembleAutoOptimize(self.handle), "acEnsembleAutoOptimize")
+
+    def box_z_depths(self):
+        """Output planes per work item of every box (0: whole columns)."""
+        return [self.lib.dll.acEnsembleBoxZDepth(self.handle, b) for b in range(self.nboxes)]
 
     def randomize_uniform(self, box, handle, lo, hi, seed):
         return self._check(self.lib.dll.acEnsembleRandomizeUniform(self.handle, box, _vh(handle), lo, hi, seed),

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 using namespace ac;
@@ -155,8 +156,10 @@ struct ensemble_s {
     size_t base_bytes;
     hipStream_t stream;
     int parity; // number of swaps mod 2: set `parity` holds the current state
-    // batched solve
+    // batched solve: one tile shape per launch; the z-depth of a box's work items is that of its
+    // size class (its column length n_z) when the tuner set one, else z_depth (0: whole columns)
     int tile_x, tile_y, z_depth, nblocks;
+    std::map<int, int> class_depth;
     BatchItem* d_items;
     int nitems;
     SubstepArgs* d_args[3][2]; // per (substep, parity), for dt_args
@@ -203,6 +206,15 @@ drop_graphs(Ensemble e)
     }
 }
 
+// Output planes per work item of a box whose columns are nz planes long.
+int
+box_depth(const Ensemble e, int nz)
+{
+    const auto it = e->class_depth.find(nz);
+    const int zd  = it != e->class_depth.end() ? it->second : e->z_depth;
+    return (zd <= 0 || zd >= nz) ? nz : zd;
+}
+
 AcResult
 build_items(Ensemble e)
 {
@@ -213,8 +225,8 @@ build_items(Ensemble e)
     std::vector<Item> items;
     const int TX = e->tile_x, TY = e->tile_y;
     for (int b = 0; b < e->nboxes; ++b) {
-        const int3 n = dims_of(e->info[b]);
-        const int zd = (e->z_depth <= 0 || e->z_depth >= n.z) ? n.z : e->z_depth;
+        const int3 n  = dims_of(e->info[b]);
+        const int zd = box_depth(e, n.z);
         const int ntx = (n.x + TX - 1) / TX, nty = (n.y + TY - 1) / TY, ntz = (n.z + zd - 1) / zd;
         for (int tz = 0; tz < ntz; ++tz) {
             const int zs    = NGHOST + tz * zd;
@@ -342,6 +354,55 @@ valid_dt(AcReal dt)
     return std::isfinite((double)dt) && dt > 0;
 }
 
+// Shortest of `samples` launches of substep 0 of the batched solve over the current work list,
+// after `warm` untimed ones. Substep 0 reads the current set and writes the other one, which
+// holds no state between steps.
+float
+time_batch(Ensemble e, int warm, int samples)
+{
+    hipEvent_t t0, t1;
+    AC_HIP_CHECK(hipEventCreate(&t0));
+    AC_HIP_CHECK(hipEventCreate(&t1));
+    const SubstepArgs* args = e->d_args[0][e->parity];
+    for (int i = 0; i < warm; ++i)
+        AC_HIP_CHECK(launch_substep_batch(e->tile_x, e->tile_y, args, e->d_items, e->nitems,
+                                          e->nblocks, e->stream));
+    float best = INFINITY;
+    for (int i = 0; i < samples; ++i) {
+        AC_HIP_CHECK(hipEventRecord(t0, e->stream));
+        AC_HIP_CHECK(launch_substep_batch(e->tile_x, e->tile_y, args, e->d_items, e->nitems,
+                                          e->nblocks, e->stream));
+        AC_HIP_CHECK(hipEventRecord(t1, e->stream));
+        AC_HIP_CHECK(hipEventSynchronize(t1));
+        float ms = 0;
+        AC_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+        best = std::min(best, ms);
+    }
+    AC_HIP_CHECK(hipEventDestroy(t0));
+    AC_HIP_CHECK(hipEventDestroy(t1));
+    return best;
+}
+
+// Tuned choices per ensemble composition (device, box sizes in order), kept for the process.
+struct EnsembleTuning {
+    int tile_x, tile_y;
+    std::map<int, int> class_depth;
+};
+
+std::map<std::vector<int>, EnsembleTuning>&
+tuning_cache()
+{
+    static std::map<std::vector<int>, EnsembleTuning> cache;
+    return cache;
+}
+
+std::mutex&
+tuning_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+
 } // namespace
 
 extern "C" {
@@ -376,8 +437,10 @@ acEnsembleCreate(const int device_id, const int nboxes, const AcMeshInfo* config
     for (auto& i : e->info)
         acHostUpdateBuiltinParams(&i);
     e->parity   = 0;
-    e->tile_x   = 32;
-    e->tile_y   = 4;
+    // 16 x 16 tiles and whole columns, the fastest measured on the 96-box workload, until the
+    // tuner (below) measures this composition
+    e->tile_x   = 16;
+    e->tile_y   = 16;
     e->z_depth  = 0;
     e->d_items  = nullptr;
     e->args_valid = false;
@@ -440,6 +503,11 @@ acEnsembleCreate(const int device_id, const int nboxes, const AcMeshInfo* config
     AC_HIP_CHECK(hipHostMalloc((void**)&e->h_results, (size_t)nboxes * NDIAG * sizeof(double)));
     if (build_items(e) != AC_SUCCESS)
         return AC_FAILURE;
+    // Tile shape and per-class z-depths measured on this composition, as Devices tune at
+    // creation; AC_AUTOTUNE=0 keeps the defaults.
+    const char* tune = getenv("AC_AUTOTUNE");
+    if (!(tune && tune[0] == '0') && acEnsembleAutoOptimize(e) != AC_SUCCESS)
+        return AC_FAILURE;
     AC_HIP_CHECK(hipDeviceSynchronize());
     *ensemble = e;
     return AC_SUCCESS;
@@ -488,8 +556,110 @@ acEnsembleSetKernelConfig(Ensemble e, const AcKernelConfig config)
     e->tile_x  = config.tile_x;
     e->tile_y  = config.tile_y;
     e->z_depth = config.z_depth;
+    e->class_depth.clear(); // one z-depth for every box
     drop_graphs(e);
     return build_items(e);
+}
+
+AcResult
+acEnsembleAutoOptimize(Ensemble e)
+{
+    if (!e)
+        return AC_FAILURE;
+    AC_HIP_CHECK(hipSetDevice(e->id));
+    AC_HIP_CHECK(hipStreamSynchronize(e->stream));
+    drop_graphs(e);
+    std::vector<int> key{e->id};
+    for (const AcMeshInfo& i : e->info) {
+        const int3 n = dims_of(i);
+        key.insert(key.end(), {n.x, n.y, n.z});
+    }
+    {
+        std::lock_guard<std::mutex> lock(tuning_mutex());
+        const auto it = tuning_cache().find(key);
+        if (it != tuning_cache().end()) {
+            e->tile_x      = it->second.tile_x;
+            e->tile_y      = it->second.tile_y;
+            e->z_depth     = 0;
+            e->class_depth = it->second.class_depth;
+            return build_items(e);
+        }
+    }
+    // the timed launches need argument arrays; any dt times the same
+    if (!e->args_valid && build_args(e, AcReal(1e-3)) != AC_SUCCESS)
+        return AC_FAILURE;
+    // 1. The launch's tile shape, every box in whole columns.
+    const int shapes[3][2] = {{16, 16}, {16, 8}, {32, 4}};
+    e->z_depth = 0;
+    e->class_depth.clear();
+    float best = INFINITY;
+    int bx = 16, by = 16;
+    for (const auto& s : shapes) {
+        e->tile_x = s[0];
+        e->tile_y = s[1];
+        if (build_items(e) != AC_SUCCESS)
+            continue;
+        const float ms = time_batch(e, 2, 3);
+        if (ms < best) {
+            best = ms;
+            bx   = s[0];
+            by   = s[1];
+        }
+    }
+    e->tile_x = bx;
+    e->tile_y = by;
+    if (build_items(e) != AC_SUCCESS)
+        return AC_FAILURE;
+    // 2. Per size class (the column length n_z), the class with the most cells first: whole
+    // columns, or columns cut into 2 or 4 z ranges of whole 7-plane rounds. A cut shortens the
+    // longest work items (the launch's tail) at the cost of 6 warm-up planes per extra item; it
+    // is kept only when the whole launch gets at least 2 % faster.
+    std::map<int, double> work;
+    for (const AcMeshInfo& i : e->info) {
+        const int3 n = dims_of(i);
+        work[n.z] += (double)n.x * n.y * n.z;
+    }
+    std::vector<std::pair<double, int>> classes;
+    for (const auto& kv : work)
+        classes.push_back({kv.second, kv.first});
+    std::sort(classes.rbegin(), classes.rend());
+    best = time_batch(e, 1, 3);
+    for (const auto& c : classes) {
+        const int nz = c.second;
+        int keep     = 0;
+        for (const int parts : {2, 4}) {
+            const int d = ((nz + parts - 1) / parts + 6) / 7 * 7;
+            if (d < 14 || d >= nz)
+                continue;
+            e->class_depth[nz] = d;
+            if (build_items(e) != AC_SUCCESS)
+                return AC_FAILURE;
+            const float ms = time_batch(e, 1, 3);
+            if (ms < 0.98f * best) {
+                best = ms;
+                keep = d;
+            }
+        }
+        if (keep > 0)
+            e->class_depth[nz] = keep;
+        else
+            e->class_depth.erase(nz);
+        if (build_items(e) != AC_SUCCESS)
+            return AC_FAILURE;
+    }
+    std::lock_guard<std::mutex> lock(tuning_mutex());
+    tuning_cache()[key] = EnsembleTuning{e->tile_x, e->tile_y, e->class_depth};
+    return AC_SUCCESS;
+}
+
+int
+acEnsembleBoxZDepth(const Ensemble e, const int box)
+{
+    if (!ens_ok(e, box))
+        return -1;
+    const int nz = dims_of(e->info[box]).z;
+    const int zd = box_depth(e, nz);
+    return zd >= nz ? 0 : zd;
 }
 
 AcResult

@@ -9,18 +9,19 @@
 //     src != dst.
 //   acNodeIntegrateSubstep: the region clipped to the interior, as node.cc:595-612 clips it to
 //     each subgrid, and skipped when empty.
-//   acNodeIntegrate / acNodeIntegrateGBC: per substep the interior is split as node.cc:729-880
-//     splits it, into the inner region [2 NGHOST, n) and six outer slabs. The inner region reads
-//     no ghost cell, so it runs on STREAM_16 while the ghost shell is filled on STREAM_0; the
-//     slabs run on STREAM_0..5 once the fill has completed, then the buffers swap. (The reference
-//     fills before it starts the inner region: what its inner region overlaps is the exchange
-//     between GPUs, which one GPU does not have.) The result is that of one launch over the
+//   acNodeIntegrate / acNodeIntegrateGBC: per substep the interior is split, as node.cc:729-880
+//     splits it, into an inner region that reads no ghost cell and an outer region that does.
+//     The inner region runs on STREAM_1 while STREAM_0 fills the ghost shell and then solves the
+//     outer region, then the buffers swap. (The reference fills before it starts the inner
+//     region: what its inner region overlaps is the exchange between GPUs, which one GPU does
+//     not have.) The outer region is cut to suit the z-march's tiles (split_step below) rather
+//     than into the reference's six 3-cell slabs. The result is that of one launch over the
 //     interior to rounding: the z-march accumulates a cell's cross-derivative sums in the phase
 //     of its unrolled plane loop that the column's first plane sets, and the phases are compiled
-//     separately, so a few cells differ in their last bits. Boxes with an axis shorter than
-//     2 NGHOST integrate the interior in one launch. Where the Device reads the ghost cells'
-//     periodic images instead of filling them (acDeviceSubstepReadsPeriodicImages), there is no
-//     fill to overlap and acNodeIntegrate runs each substep as one such launch.
+//     separately, so a few cells differ in their last bits. Boxes no more than two tiles wide in
+//     x or y, or 2 NGHOST deep in z, integrate the interior in one launch. Where the Device reads
+//     the ghost cells' periodic images (acDeviceSubstepReadsPeriodicImages), there is no fill to
+//     overlap and acNodeIntegrate runs each substep as one such launch.
 //   acNodeReduce*: the Device reduction, finalised as node.cc:932-957 finalises the per-GPU
 //     results: RMS, RMS_EXP and ALFVEN_RMS return sqrt(sum / (n_x n_y n_z)).
 //   acNodePrintInfo, acNodeAutoOptimize: the Device's (the reference returns AC_FAILURE for both,
@@ -84,6 +85,16 @@ finalise(const Node node, ReductionType rtype, AcReal sum)
 
 // The step of node.cc:729-880 on one Device; `fill` fills the ghost shell of the `in` buffers
 // on the given stream.
+//
+// The reference's outer region is six slabs NGHOST cells thick. Solved by tiles of the z-march
+// (16 x 16 cells by default), a slab 3 cells thin in x or y leaves 13 of 16 lanes of every row
+// idle, and a 3-plane z slab marches 6 warm-up planes per 3 outputs: that made the split step
+// 34 % slower than one launch over the interior. Here the x-y part of the outer region is a ring
+// one tile wide (a_x = max(NGHOST, tile_x) cells, a_y likewise), solved over all of z, and only
+// the inner x-y part keeps the two z slabs; the inner region [N + a_x, n + N - a_x) x
+// [N + a_y, n + N - a_y) x [2N, n) still reads no ghost cell. The inner region runs on STREAM_1
+// while STREAM_0 fills the ghost shell and then, in stream order, solves the ring (4 boxes) and
+// the two z slabs, so no host wait separates the fill from the outer region.
 template <typename Fill>
 AcResult
 split_step(const Node node, const AcReal dt, Fill fill)
@@ -91,7 +102,18 @@ split_step(const Node node, const AcReal dt, Fill fill)
     const Device d = node->device;
     const int3 n   = node->grid.n;
     const int N    = NGHOST;
-    const bool split = n.x >= 2 * N && n.y >= 2 * N && n.z >= 2 * N;
+    AcKernelConfig whole;
+    if (acDeviceGetKernelConfig(d, &whole) != AC_SUCCESS)
+        return AC_FAILURE;
+    const int ax = whole.tile_x > N ? whole.tile_x : N;
+    const int ay = whole.tile_y > N ? whole.tile_y : N;
+    const bool split = n.x > 2 * ax && n.y > 2 * ay && n.z > 2 * N;
+    // Ring columns span the whole depth in 28-plane z ranges, so that each ring launch has
+    // enough blocks to spread over the GPU beside the inner region (a whole-depth column per
+    // tile leaves a 512-cell side of the ring 32 blocks).
+    AcKernelConfig ring = whole;
+    if (ring.variant != 0)
+        ring.z_depth = 28;
     if (acDeviceSynchronizeStream(d, STREAM_ALL) != AC_SUCCESS)
         return AC_FAILURE;
     for (int s = 0; s < 3; ++s) {
@@ -103,38 +125,29 @@ split_step(const Node node, const AcReal dt, Fill fill)
                 return AC_FAILURE;
         }
         else {
+            const int x0 = N + ax, x1 = n.x + N - ax, y0 = N + ay, y1 = n.y + N - ay;
             // inner region: reads cells [N, n + N) only, concurrently with the fill
-            if (acDeviceIntegrateSubstep(d, STREAM_16, s, i3(2 * N, 2 * N, 2 * N), n, dt) !=
+            if (acDeviceIntegrateSubstep(d, STREAM_1, s, i3(x0, y0, 2 * N), i3(x1, y1, n.z), dt) !=
                 AC_SUCCESS)
                 return AC_FAILURE;
-            if (acDeviceSynchronizeStream(d, STREAM_0) != AC_SUCCESS) // the ghost shell is filled
-                return AC_FAILURE;
-            // The slabs are thin: z-march columns 14 planes deep give their launches enough
-            // blocks to spread over the GPU (a whole-depth column per 16 x 16 tile leaves a side
-            // slab of a 512^3 box 32 blocks). The inner launch has already taken its own depth.
-            AcKernelConfig whole;
-            if (acDeviceGetKernelConfig(d, &whole) != AC_SUCCESS)
-                return AC_FAILURE;
-            AcKernelConfig thin = whole;
-            if (thin.variant != 0)
-                thin.z_depth = 14;
-            if (acDeviceSetKernelConfig(d, thin) != AC_SUCCESS)
-                return AC_FAILURE;
             const struct {
-                Stream stream;
+                bool ring;
                 int3 start, end;
-            } slabs[6] = {
-                {STREAM_0, i3(N, N, N), i3(n.x + N, n.y + N, 2 * N)},            // front
-                {STREAM_1, i3(N, N, n.z), i3(n.x + N, n.y + N, n.z + N)},        // back
-                {STREAM_2, i3(N, N, 2 * N), i3(n.x + N, 2 * N, n.z)},            // bottom
-                {STREAM_3, i3(N, n.y, 2 * N), i3(n.x + N, n.y + N, n.z)},        // top
-                {STREAM_4, i3(N, 2 * N, 2 * N), i3(2 * N, n.y, n.z)},            // left
-                {STREAM_5, i3(n.x, 2 * N, 2 * N), i3(n.x + N, n.y, n.z)},        // right
+            } outer[6] = {
+                {true, i3(N, N, N), i3(n.x + N, y0, n.z + N)},        // ring, low y
+                {true, i3(N, y1, N), i3(n.x + N, n.y + N, n.z + N)},  // ring, high y
+                {true, i3(N, y0, N), i3(x0, y1, n.z + N)},            // ring, low x
+                {true, i3(x1, y0, N), i3(n.x + N, y1, n.z + N)},      // ring, high x
+                {false, i3(x0, y0, N), i3(x1, y1, 2 * N)},            // inner x-y, low z
+                {false, i3(x0, y0, n.z), i3(x1, y1, n.z + N)},        // inner x-y, high z
             };
             AcResult r = AC_SUCCESS;
-            for (const auto& sl : slabs)
+            for (const auto& o : outer) {
                 if (r == AC_SUCCESS)
-                    r = acDeviceIntegrateSubstep(d, sl.stream, s, sl.start, sl.end, dt);
+                    r = acDeviceSetKernelConfig(d, o.ring ? ring : whole);
+                if (r == AC_SUCCESS)
+                    r = acDeviceIntegrateSubstep(d, STREAM_0, s, o.start, o.end, dt);
+            }
             if (acDeviceSetKernelConfig(d, whole) != AC_SUCCESS || r != AC_SUCCESS)
                 return AC_FAILURE;
         }

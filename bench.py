@@ -275,25 +275,64 @@ def measured_traffic(prec: str, n: int, cfg: dict):
     return None, None
 
 
-def cpu_baseline(n=128, steps=3):
-    """The oracle (CPU restatement of the host model solver, FP64, OpenMP over k) on the host."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline():
+    """The oracle (the CPU restatement of the host model solver, acHostIntegrateStep's form) on the
+    host cores: one RK3 step of the uniform-field box at 64^3 (FP64 and long double) and 256^3
+    (FP64), single-threaded (the reference's host solver runs on one thread, its OpenMP pragmas
+    commented out at modelsolver.c:1018,1028) and on all the cores this process may use (OpenMP
+    over k planes, at most the thread budget the environment sets). `value` is the all-core
+    256^3 FP64 rate."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import numpy as np
     import pyoracle
     from astaroth_amd.api import Library, real_params_double
     from astaroth_amd.initial import mhd_fields
 
-    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16)
-    threads = pyoracle.set_threads(threads)
-    info = Library(True).config(n=n)
-    fields = mhd_fields(n, n, n, np.float64, config_id=1)
-    rp = real_params_double(info)
-    pyoracle.integrate_steps([f.copy() for f in fields], rp, 1e-3, 1)  # warm
-    t0 = time.perf_counter()
-    pyoracle.integrate_steps(fields, rp, 1e-3, steps)
-    dt = time.perf_counter() - t0
-    return dict(value=n**3 * steps / dt, unit="cell-updates/s", cores=threads, kind="port",
-                sample="%d^3 uniform-field box, %d RK3 step(s), FP64 host-form model solver, %.2f s" % (n, steps, dt))
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    budget = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or usable
+    all_cores = max(1, min(usable, budget))
+    runs = []
+
+    initial = {}
+
+    def timed(n, dtype, threads):
+        threads = pyoracle.set_threads(threads)
+        if n not in initial:
+            initial.clear()
+            initial[n] = mhd_fields(n, n, n, np.float64, config_id=1)
+        fields = [f.astype(dtype) for f in initial[n]]
+        rp = real_params_double(Library(True).config(n=n))
+        t0 = time.perf_counter()
+        pyoracle.integrate_steps(fields, rp, 1e-3, 1)
+        dt = time.perf_counter() - t0
+        del fields
+        runs.append({"n": n, "dtype": np.dtype(dtype).name, "threads": threads, "steps": 1, "s": dt,
+                     "cell_updates_per_s": n ** 3 / dt})
+        return runs[-1]
+
+    timed(32, np.float64, all_cores)  # untimed in effect: first touch of the thread pool
+    runs.clear()
+    for threads in (1, all_cores):
+        timed(64, np.float64, threads)
+        timed(64, np.longdouble, threads)
+    best = timed(256, np.float64, all_cores)
+    single = timed(256, np.float64, 1)
+    return dict(value=best["cell_updates_per_s"], unit="cell-updates/s", cores=best["threads"], kind="port",
+                sample="256^3 uniform-field box, 1 RK3 step, FP64 host-form model solver (oracle/), %d threads, "
+                       "%.2f s" % (best["threads"], best["s"]),
+                single_thread=single["cell_updates_per_s"], nproc=os.cpu_count(), usable_cores=usable,
+                cpu_model=cpu_model(), runs=runs)
 
 
 def main(argv=None):
@@ -313,12 +352,15 @@ def main(argv=None):
                     help="force a solve configuration 'variant,tile_x,tile_y,z_depth' instead of autotuning")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="REPS",
                     help="only run REPS launches of the plain copy kernel and print its bandwidth")
-    ap.add_argument("--workload", choices=["headline", "sweep256", "precision512", "ensemble96", "halo1024"],
+    ap.add_argument("--workload", choices=["headline", "sweep256", "precision512", "ensemble96", "halo1024",
+                                           "reductions", "boundaries", "node"],
                     default="headline",
                     help="headline: the 512^3 FP32 vs FP64 metric (default). sweep256: 256^3 with per-step "
                          "FP64 diagnostics and the tuning sweep. precision512: 512^3 FP32-vs-FP64 precision "
                          "study. ensemble96: 96-box FP32 ensemble, batched vs looped. halo1024: 1024^3 FP32 with "
-                         "the 26-block pack/unpack halo")
+                         "the 26-block pack/unpack halo. reductions: MAX/RMS of lnrho and |u| at --n (512^3). "
+                         "boundaries: periodic fill and 26-block pack/unpack at --n (512^3). node: "
+                         "acNodeIntegrate against acDeviceIntegrateStep, FP32, at --n (512^3)")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this path has no multi-GPU layer (single periodic box "
@@ -450,6 +492,31 @@ def run_workload(a, kcfg, hip):
         line["vs_looped"] = best["value"] / r["looped"]["value"]
         print(json.dumps(line), flush=True)
         return 0
+    if a.workload == "reductions":
+        r = reductions_benchmark(a.n, hip)
+        best = max(r, key=lambda x: x["gbs"])
+        line = dict(base, metric="HBM GB/s of the FP64-accumulated reductions at %d^3" % a.n, value=best["gbs"],
+                    unit="GB/s", steps=5, warmup=1, dtype="f32/f64 data, f64 accumulation",
+                    config={"workload": "MAX and RMS of lnrho (1 field) and |u| (3 fields), %d^3 uniform-field box"
+                                        % a.n}, results=r)
+        print(json.dumps(line), flush=True)
+        return 0
+    if a.workload == "node":
+        r = node_benchmark(a.n, a.dt)
+        line = dict(base, metric="cell-updates/s per RK3 step of acNodeIntegrate at %d^3 FP32" % a.n,
+                    value=a.n ** 3 / (r["node_ms_per_step"] * 1e-3), unit="cell-updates/s", steps=5, warmup=1,
+                    ms_per_step=r["node_ms_per_step"], dtype="f32",
+                    config={"workload": "8-field MHD, %d^3 periodic box, Node step (inner region during the fill, "
+                                        "then the outer region) against the Device step" % a.n}, results=r)
+        print(json.dumps(line), flush=True)
+        return 0
+    if a.workload == "boundaries":
+        out = {prec: boundaries_benchmark(a.n, hip, prec == "fp64") for prec in ("fp32", "fp64")}
+        line = dict(base, metric="HBM GB/s of the periodic fill and the 26-block pack/unpack at %d^3" % a.n,
+                    value=out["fp32"]["periodic_fill"]["gbs"], unit="GB/s", steps=10, warmup=1, dtype="f32",
+                    config={"workload": "ghost shell of 8 fields, %d^3 periodic box" % a.n}, results=out)
+        print(json.dumps(line), flush=True)
+        return 0
     if a.workload == "sweep256":
         n = 256 if a.n == 512 else a.n
         out = {}
@@ -479,7 +546,11 @@ def ensemble_benchmark(dt, hip, steps=10, warmup=2):
     infos = [lib.config(n=b[0], ny=b[1], nz=b[2], AC_nu_visc=nu) for b, nu in zip(boxes, nus)]
     cells = float(sum(b[0] * b[1] * b[2] for b in boxes))
     out = {"boxes": boxes, "cells": cells, "steps": steps, "warmup": warmup}
-    with lib.ensemble_create(infos) as ens:
+    t0 = time.perf_counter()
+    with lib.ensemble_create(infos, autotune=True) as ens:  # the library's tuner runs at creation
+        out["autotune_s"] = time.perf_counter() - t0
+        tuned = ens.kernel_config()
+        depths = ens.box_z_depths()
         stream = ens.stream_handle()
 
         def timed(mode, nsteps):
@@ -497,15 +568,18 @@ def ensemble_benchmark(dt, hip, steps=10, warmup=2):
             ms = hip.elapsed(e0, e1) / nsteps
             return {"ms_per_step": ms, "value": cells / (ms * 1e-3)}
 
-        # tile / z-depth choice for the batched launch (the ensemble's own tuning pass)
-        tuning = []
+        # fixed configurations for every box, beside the tuner's choice
+        sweep = []
         for cfg in ((32, 4, 0), (32, 4, 28), (16, 8, 0), (16, 8, 28), (16, 16, 0), (16, 16, 28)):
             ens.set_kernel_config(*cfg)
-            tuning.append([list(cfg), timed("batched", 3)["ms_per_step"]])
-        tuning.sort(key=lambda x: x[1])
-        ens.set_kernel_config(*tuning[0][0])
-        out["tuning"] = tuning
-        out["kernel_config"] = tuning[0][0]
+            sweep.append([list(cfg), timed("batched", 3)["ms_per_step"]])
+        sweep.sort(key=lambda x: x[1])
+        ens.autotune()  # back to the tuned choice (kept for this composition)
+        assert ens.kernel_config().as_tuple() == tuned.as_tuple() and ens.box_z_depths() == depths
+        out["fixed_config_sweep"] = sweep
+        out["kernel_config"] = [tuned.tile_x, tuned.tile_y]
+        out["z_depth_per_size_class"] = {str(b[2]): d for b, d in zip(boxes, depths)}
+        out["tuned_batched"] = timed("batched", 3)["ms_per_step"]
         out["batched_eager"] = timed("batched", steps)
         out["batched_graph"] = timed("graph", steps)
         out["looped"] = timed("looped", steps)
@@ -514,6 +588,112 @@ def ensemble_benchmark(dt, hip, steps=10, warmup=2):
             raise RuntimeError("ensemble state diverged")
         out["diagnostics_max_u_range"] = [float(d[:, 0].min()), float(d[:, 0].max())]
     return out
+
+
+def reductions_benchmark(n: int, hip: Hip, reps=5):
+    """acDeviceReduceScalDouble / acDeviceReduceVecDouble, MAX and RMS of lnrho and of |u|, `reps`
+    calls each on an FP32 and an FP64 Device holding the uniform-field box; each call is bracketed
+    by HIP events on the library's stream (both launches and the 8-byte result copy). Algorithmic
+    bytes: fields read x n^3 x sizeof(AcReal)."""
+    from astaroth_amd.api import Library
+    from astaroth_amd.initial import device_init_mhd
+
+    out = []
+    for double in (False, True):
+        lib = Library(double)
+        with lib.device_create(0, lib.config(n=n), autotune=False) as dev:
+            device_init_mhd(dev)
+            stream = dev.stream_handle()
+            calls = [("scalar", "lnrho", 1, lambda r: dev.reduce_scal(r, "VTXBUF_LNRHO", double=True)),
+                     ("vector", "|u|", 3, lambda r: dev.reduce_vec(r, "VTXBUF_UUX", "VTXBUF_UUY", "VTXBUF_UUZ",
+                                                                   double=True))]
+            for kind, what, nfields, fn in calls:
+                for rtype in ("RTYPE_MAX", "RTYPE_RMS"):
+                    fn(rtype)  # warm
+                    ms = []
+                    for _ in range(reps):
+                        a, b = hip.event(), hip.event()
+                        hip.record(a, stream)
+                        fn(rtype)
+                        hip.record(b, stream)
+                        hip.sync()
+                        ms.append(hip.elapsed(a, b))
+                    nbytes = float(nfields) * n ** 3 * (8 if double else 4)
+                    med = statistics.median(ms)
+                    out.append({"build": "fp64" if double else "fp32", "reduction": "%s %s %s" % (kind, rtype, what),
+                                "fields": nfields, "median_ms": med, "bytes": nbytes,
+                                "gbs": nbytes / (med * 1e-3) / 1e9})
+                    print("# %-4s %-22s %.1f us, %.0f GB/s" % (out[-1]["build"], out[-1]["reduction"], med * 1e3,
+                                                              out[-1]["gbs"]), file=sys.stderr, flush=True)
+    return out
+
+
+def node_benchmark(n: int, dt: float, steps=5, rounds=3):
+    """acNodeIntegrate (inner region during the ghost fill, then the outer region) against
+    acDeviceIntegrateStep on the same n^3 FP32 state, alternating `rounds` times between the two
+    so that both see the same clocks; host wall time per RK3 step (both are synchronous calls)."""
+    from astaroth_amd.api import Device, Library
+    from astaroth_amd.initial import device_init_mhd
+
+    lib = Library(False)
+    info = lib.config(n=n)
+    node_ms, dev_ms = [], []
+    with lib.node_create(info, autotune=False) as node:
+        # the Node's own Device, driven through the Device calls
+        dev = Device.view(lib, info, node.query_device_configuration().devices[0])
+        device_init_mhd(dev)
+        images = lib.dll.acDeviceSubstepReadsPeriodicImages(dev.handle) == 1
+        node.integrate(dt)
+        dev.integrate_step(dt)
+        dev.synchronize()
+        for _ in range(rounds):
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                node.integrate(dt)
+            node_ms.append((time.perf_counter() - t0) * 1e3 / steps)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                dev.integrate_step(dt)
+            dev.synchronize()
+            dev_ms.append((time.perf_counter() - t0) * 1e3 / steps)
+            print("# node %.2f ms/step, device %.2f ms/step" % (node_ms[-1], dev_ms[-1]), file=sys.stderr, flush=True)
+    return {"node_ms_per_step": min(node_ms), "device_ms_per_step": min(dev_ms),
+            "node_over_device": min(node_ms) / min(dev_ms), "device_reads_images": images,
+            "all_node_ms": node_ms, "all_device_ms": dev_ms}
+
+
+def boundaries_benchmark(n: int, hip: Hip, double: bool, reps=10):
+    """The periodic ghost fill (acDevicePeriodicBoundconds) and the 26-block self exchange
+    (acDeviceHaloExchangeSelf: pack + unpack), `reps` launches each, timed with HIP events on the
+    library's stream. Algorithmic bytes: the ghost shell g = m^3 - n^3 cells of 8 fields, read and
+    written once by the fill (2 x 8 g s); read, packed, unpacked and written by the exchange
+    (4 x 8 g s)."""
+    from astaroth_amd.api import Library
+    from astaroth_amd.initial import device_init_mhd
+
+    lib = Library(double)
+    s = 8 if double else 4
+    g = float((n + 6) ** 3 - n ** 3)
+    res = {}
+    with lib.device_create(0, lib.config(n=n), autotune=False) as dev:
+        device_init_mhd(dev)
+        stream = dev.stream_handle()
+        for name, fn, nbytes in (("periodic_fill", dev.periodic_boundconds, 2 * 8 * g * s),
+                                 ("pack_unpack_26", dev.halo_exchange_self, 4 * 8 * g * s)):
+            fn()
+            dev.synchronize()
+            a, b = hip.event(), hip.event()
+            hip.record(a, stream)
+            for _ in range(reps):
+                fn()
+            hip.record(b, stream)
+            hip.sync()
+            ms = hip.elapsed(a, b) / reps
+            res[name] = {"ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9,
+                         "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            print("# %s %d^3 %s: %.1f us, %.0f GB/s" % ("fp64" if double else "fp32", n, name, ms * 1e3,
+                                                      res[name]["gbs"]), file=sys.stderr, flush=True)
+    return res
 
 
 def tuning_sweep(double: bool, n: int):

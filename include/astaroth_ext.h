@@ -170,9 +170,18 @@ AcResult acEnsembleCreate(const int device_id, const int nboxes, const AcMeshInf
                           Ensemble* ensemble);
 AcResult acEnsembleDestroy(Ensemble ensemble);
 int acEnsembleBoxCount(const Ensemble ensemble);
-/* Batched solve configuration: variant 3, tile 32x4, 16x8 or 16x16, z_depth (0: whole box). */
+/* Batched solve configuration: variant 3, tile 32x4, 16x8 or 16x16, z_depth (0: whole box) for
+ * every box (it replaces the tuner's per-class z-depths). */
 AcResult acEnsembleSetKernelConfig(Ensemble ensemble, const AcKernelConfig config);
 AcResult acEnsembleGetKernelConfig(const Ensemble ensemble, AcKernelConfig* config);
+/* The ensemble's tuner, run by acEnsembleCreate unless the environment sets AC_AUTOTUNE=0 (then
+ * 16 x 16 tiles and whole columns): times the batched launch over the whole work list for each
+ * tile shape, then, per size class (boxes of one column length n_z, the class with the most cells
+ * first), whole columns against columns cut into 2 or 4 z ranges, keeping a cut only when the
+ * launch gets 2 % faster. Results are kept per composition (device and box sizes) for the
+ * process. acEnsembleBoxZDepth gives a box's output planes per work item (0: whole columns). */
+AcResult acEnsembleAutoOptimize(Ensemble ensemble);
+int acEnsembleBoxZDepth(const Ensemble ensemble, const int box);
 AcResult acEnsembleRandomizeUniform(Ensemble ensemble, const int box,
                                     const VertexBufferHandle handle, const double lo,
                                     const double hi, const unsigned long long seed);

@@ -2,14 +2,16 @@
 through the C ABI.
 
 * acNodeIntegrate (with its Device filling ghost cells) runs each substep as the inner region
-  [2 NGHOST, n) on STREAM_16 while the ghost shell is filled on STREAM_0, then the six outer slabs
-  on STREAM_0..5; its interior equals
-  that of acDeviceIntegrateStep (one launch over the interior per substep) to rounding (range-
+  (x and y one tile in from the faces, z in [2 NGHOST, n)) on STREAM_1 while STREAM_0 fills the
+  ghost shell and then solves the outer region: a one-tile ring in x and y over all of z (in
+  28-plane z ranges) and the two z slabs of the inner x-y part. Its interior equals that of
+  acDeviceIntegrateStep (one launch over the interior per substep) to rounding (range-
   normalised 2e-6 in FP32, 1e-13 in FP64, the bar between kernel configurations: a cell's
   cross-derivative sums are accumulated in the plane-loop phase that its column's start sets,
-  and the phases are compiled separately, so a few cells differ in their last bits), and bit for
-  bit on a box with an axis shorter than 2 NGHOST, where the Node integrates in one launch, and
-  where the Device reads the ghost cells' periodic images (one launch per substep, no fill).
+  and the phases are compiled separately, so a few cells differ in their last bits), on boxes
+  with and without several ring z ranges and with partial tiles; and bit for bit where the box
+  is two tiles wide or less (one launch), and where the Device reads the ghost cells' periodic
+  images (one launch per substep, no fill).
   acNodeIntegrateGBC with symmetric / antisymmetric faces equals acDeviceIntegrateStepGBC to the
   same bar, ghost cells included.
 * acNodeIntegrateSubstep clips its region to the interior.
@@ -52,10 +54,12 @@ def host_mesh(lib, info, seed_config=1):
     return HostMesh(lib, info, mhd_fields(nx, ny, nz, np.float64, config_id=seed_config))
 
 
-@pytest.mark.parametrize("dims,images", [((24, 20, 18), 0), ((24, 20, 18), 1), ((24, 5, 18), 0),
-                                         ((24, 20, 40), 0)],
-                         ids=["split", "images", "thin-y", "multi-column-slabs"])
-def test_node_integrate_equals_device_step(lib, dims, images):
+@pytest.mark.parametrize("dims,images,split", [((24, 20, 18), 0, False), ((24, 20, 18), 1, False),
+                                               ((24, 5, 18), 0, False), ((48, 40, 18), 0, True),
+                                               ((40, 36, 64), 0, True), ((34, 70, 9), 0, True)],
+                         ids=["narrower-than-two-tiles", "images", "thin-y", "ring", "ring-multi-column",
+                              "ring-odd"])
+def test_node_integrate_equals_device_step(lib, dims, images, split):
     info = lib.config(n=dims[0], ny=dims[1], nz=dims[2])
     mesh = host_mesh(lib, info)
     with lib.node_create(info, autotune=False) as node:
@@ -71,7 +75,7 @@ def test_node_integrate_equals_device_step(lib, dims, images):
         for _ in range(2):
             dev.integrate_step(1e-3)
         ref = dev.store_mesh(mesh.copy())
-    if dims[1] < 6 or images:  # one launch over the interior: the same arithmetic, cell for cell
+    if not split or images:  # one launch over the interior: the same arithmetic, cell for cell
         for q in range(8):
             assert np.array_equal(got.interior(q), ref.interior(q)), abi.VTXBUF_NAMES[q]
     else:
