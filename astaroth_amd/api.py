@@ -113,6 +113,7 @@ class Library:
             "acDeviceReduceScalDouble": (i, [vp, i, i, i, ctypes.POINTER(ctypes.c_double)]),
             "acDeviceReduceVecDouble": (i, [vp, i, i, i, i, i, ctypes.POINTER(ctypes.c_double)]),
             "acDeviceReduceVecScalDouble": (i, [vp, i, i, i, i, i, i, ctypes.POINTER(ctypes.c_double)]),
+            "acDeviceReduceNoAlloc": (i, [vp, i, i, i, ctypes.POINTER(i), vp]),
             "acDeviceRunMPITest": (i, []),
             "acNodeCreate": (i, [i, MI, ctypes.POINTER(vp)]),
             "acNodeDestroy": (i, [vp]),

@@ -131,6 +131,10 @@ hipError_t launch_reduce(int rtype, int nvals, const AcReal* const* vals, const 
                          double* d_partials, int max_partials, double* d_result,
                          hipStream_t stream);
 int reduce_partials_needed();
+// Partials capacity (FP64 values) of the library's per-GPU reduction scratch array, and its
+// device address on `device_id`, which must be the current device (looked up once per device).
+int reduce_scratch_capacity();
+hipError_t reduce_scratch_address(int device_id, double** out);
 
 // Device-side PRNG fill (init.hip): value = lo + (hi - lo) * u (+ amp sin(kdz (k - 3))),
 // u = splitmix64(seed, cell) as a double in [0, 1), over the interior [n_min, n_max) of one
@@ -161,6 +165,7 @@ struct device_s {
     hipStream_t streams[NUM_STREAMS];
     ac::VertexBufferArray vba;
     double* reduce_partials;
+    double* reduce_scratch; // this GPU's copy of the library's reduction scratch array
     double* reduce_result;
     double* reduce_result_host; // pinned
     double* diag_partials;      // acDeviceDiagnostics: per-block partials, results, pinned copy

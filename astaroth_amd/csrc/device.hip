@@ -512,6 +512,8 @@ acDeviceCreate(const int id, const AcMeshInfo config, Device* device_handle)
     AC_HIP_CHECK(hipMalloc((void**)&d->diag_result, 9 * sizeof(double)));
     AC_HIP_CHECK(hipHostMalloc((void**)&d->diag_result_host, 9 * sizeof(double), hipHostMallocDefault));
     AC_HIP_CHECK(hipStreamSynchronize(d->streams[0]));
+    // this GPU's copy of the reduction scratch array (acDeviceReduceNoAlloc), with `id` current
+    AC_HIP_CHECK(reduce_scratch_address(id, &d->reduce_scratch));
 
     acDeviceLoadDefaultUniforms(d);
     acDeviceLoadMeshInfo(d, config);
@@ -1032,6 +1034,55 @@ acDeviceReduceVecScalDouble(const Device device, const Stream stream, const Redu
 {
     const int h[4] = {(int)vtxbuf0, (int)vtxbuf1, (int)vtxbuf2, (int)vtxbuf3};
     return reduce_impl(device, stream, rtype, 4, h, result);
+}
+
+// The reductions of acDeviceReduce{Scal,Vec,VecScal}Double with the same two launches, their
+// partials in the library's per-GPU scratch array instead of the Device's allocation, and the
+// FP64 result written to the caller's device memory without waiting: nothing is allocated and
+// nothing blocks the host. Any grid size fits: the row walk is grid-stride and the first launch
+// writes at most reduce_scratch_capacity() partials.
+AcResult
+acDeviceReduceNoAlloc(const Device device, const Stream stream, const ReductionType rtype,
+                      const int num_fields, const VertexBufferHandle* fields, double* d_result)
+{
+    if (!device || !fields || !d_result) {
+        fprintf(stderr, "acDeviceReduceNoAlloc: null device, field list or result pointer\n");
+        return AC_FAILURE;
+    }
+    if (num_fields != 1 && num_fields != 3 && num_fields != 4) {
+        fprintf(stderr, "acDeviceReduceNoAlloc: %d fields; 1 (scalar), 3 (vector) or 4 (Alfven) expected\n",
+                num_fields);
+        return AC_FAILURE;
+    }
+    if (!stream_ok(device, stream)) {
+        fprintf(stderr, "acDeviceReduceNoAlloc: invalid stream %d\n", stream);
+        return AC_FAILURE;
+    }
+    const bool alfven = rtype == RTYPE_ALFVEN_MAX || rtype == RTYPE_ALFVEN_MIN || rtype == RTYPE_ALFVEN_RMS;
+    const bool plain  = rtype == RTYPE_MAX || rtype == RTYPE_MIN || rtype == RTYPE_RMS ||
+                       rtype == RTYPE_RMS_EXP || rtype == RTYPE_SUM;
+    if (num_fields == 4 ? !alfven : !plain) {
+        fprintf(stderr, "acDeviceReduceNoAlloc: reduction type %d does not apply to %d field(s)\n",
+                (int)rtype, num_fields);
+        return AC_FAILURE;
+    }
+    const AcReal* vals[4];
+    for (int q = 0; q < num_fields; ++q) {
+        if (fields[q] < 0 || fields[q] >= NF) {
+            fprintf(stderr, "acDeviceReduceNoAlloc: invalid VertexBufferHandle %d\n", (int)fields[q]);
+            return AC_FAILURE;
+        }
+        vals[q] = device->vba.in[fields[q]];
+    }
+    AC_HIP_CHECK(hipSetDevice(device->id));
+    const hipError_t e = launch_reduce((int)rtype, num_fields, vals, device->local_config,
+                                       device->reduce_scratch, reduce_scratch_capacity(), d_result,
+                                       device->streams[stream]);
+    if (e != hipSuccess) {
+        fprintf(stderr, "acDeviceReduceNoAlloc: launch failed: %s\n", hipGetErrorString(e));
+        return AC_FAILURE;
+    }
+    return AC_SUCCESS;
 }
 
 AcResult

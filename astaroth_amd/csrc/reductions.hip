@@ -14,6 +14,15 @@
 // and compared exactly; sums accumulate in FP64.
 #include "ac_internal.h"
 
+#include <mutex>
+
+// Per-block partials of the allocation-free reduction (acDeviceReduceNoAlloc): 4096 FP64 values
+// (32 KiB), the most any reduction launch writes, since its block count is capped there. Each GPU
+// holds its own copy in the library's code object until the process ends. In host code the name
+// is a stand-in, not a device address: host code uses the address reduce_scratch_address looks
+// up, never `scratch` itself.
+static __device__ __align__(256) double scratch[4096];
+
 namespace ac {
 
 namespace {
@@ -21,6 +30,7 @@ namespace {
 constexpr int RED_THREADS  = 256;
 constexpr int RED_WAVES    = RED_THREADS / 64;
 constexpr int MAX_PARTIALS = 4096;
+static_assert(sizeof(scratch) == MAX_PARTIALS * sizeof(double), "scratch holds one launch's partials");
 
 enum Op { OP_MAX, OP_MIN, OP_SUM };
 
@@ -291,6 +301,33 @@ int
 reduce_partials_needed()
 {
     return MAX_PARTIALS;
+}
+
+int
+reduce_scratch_capacity()
+{
+    return MAX_PARTIALS;
+}
+
+hipError_t
+reduce_scratch_address(int device_id, double** out)
+{
+    // one lookup per GPU: each holds its own copy of the array at its own address
+    static std::mutex mu;
+    static std::map<int, double*> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    const auto it = cache.find(device_id);
+    if (it != cache.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    void* p            = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+        return e;
+    cache[device_id] = static_cast<double*>(p);
+    *out             = static_cast<double*>(p);
+    return hipSuccess;
 }
 
 hipError_t

@@ -26,6 +26,23 @@ AcResult acDeviceReduceVecScalDouble(const Device device, const Stream stream,
                                      const VertexBufferHandle vtxbuf2,
                                      const VertexBufferHandle vtxbuf3, double* result);
 
+/* ---- Allocation-free reduction ----
+ * No counterpart in the reference. For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. The reduction of
+ * acDeviceReduceScalDouble (num_fields = 1), acDeviceReduceVecDouble (3) or
+ * acDeviceReduceVecScalDouble (4, the Alfven types) over fields[0..num_fields), with the same
+ * FP64 result bit for bit (RMS types: the raw sum, as at Device level), written to d_result, a
+ * device pointer the caller owns, on `stream` without waiting. Its per-block partials live in a
+ * 32 KiB array of the library's code object, one per GPU (any grid size fits: the walk is
+ * grid-stride and writes at most 4096 partials), instead of memory the Device allocated. That
+ * array is shared by every call on the GPU and keeps the last call's partials: calls that overlap
+ * on one GPU (several streams or Devices without ordering between them) race. Returns
+ * AC_FAILURE for a null pointer, a field count other than 1, 3 or 4, a reduction type that does
+ * not apply to that count, or an invalid stream or handle. */
+AcResult acDeviceReduceNoAlloc(const Device device, const Stream stream, const ReductionType rtype,
+                               const int num_fields, const VertexBufferHandle* fields,
+                               double* d_result);
+
 /* ---- Whole RK3 step ----
  * The interior result of three times {periodic fill of `in` over the whole m^3 box, substep over
  * the interior, swap}, the caller sequence of the reference's single-GPU node loop
