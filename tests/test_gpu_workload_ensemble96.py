@@ -97,6 +97,8 @@ def test_ensemble96_against_oracle(oracle_boxes, double):
     rel = 1e-12 if double else 1e-5
     worst = {}
     for b, shape in enumerate(BOXES):
+        if b % 16 == 0:
+            print("checking boxes %d-%d" % (b, min(b + 15, len(BOXES) - 1)), flush=True)
         for q in range(8):
             if double:
                 ok, v = fp64_verdict(ref[b][q], got[b][q], q, STEPS, DT, CONFIG_ID, shape, box_id=b,

@@ -54,3 +54,18 @@ def test_long_double_cone_slice_equals_whole_box():
         iz = (np.arange(3) + centre[2] - 1) % nzw
         assert np.array_equal(sl[q], w[np.ix_(iz, iy, ix)]), q
         assert np.array_equal(window(whole[q].astype(np.float64), centre, 3), w[np.ix_(iz, iy, ix)].astype(np.float64))
+
+
+def test_verify_field_fast_path_equals_long_double_path():
+    """verify_field finds the max-abs-error point on FP64 differences and re-checks the near
+    maxima in long double; the result equals the all-long-double evaluation."""
+    rng = np.random.default_rng(3)
+    model = rng.uniform(-1e-3, 1e-3, 20000)
+    for cand in (model + rng.integers(-6, 7, model.size) * np.spacing(np.abs(model)),
+                 model.astype(np.float32).astype(np.float64),
+                 model * (1 + 1e-9 * rng.standard_normal(model.size))):
+        for c in (cand, cand.astype(np.float32)) if cand.dtype == np.float64 else (cand,):
+            fast = pyoracle.verify_field(model, c, 53, np.finfo(np.float64).eps)
+            slow = pyoracle.verify_field(model.astype(np.longdouble), c.astype(np.longdouble), 53,
+                                         np.finfo(np.float64).eps)
+            assert fast == slow, (fast, slow)
