@@ -37,7 +37,7 @@ periodic_fill_kernel(const FillArgs a)
     if (t >= a.g.cells())
         return;
     int i, j, k;
-    a.g.cell(t, i, j, k);
+    a.g.cell((unsigned)t, i, j, k);
     if (i < a.start.x || j < a.start.y || k < a.start.z || i >= a.end.x || j >= a.end.y ||
         k >= a.end.z)
         return;
@@ -86,7 +86,7 @@ general_fill_kernel(const GeneralArgs a)
     if (t >= a.g.cells())
         return;
     int i, j, k;
-    a.g.cell(t, i, j, k);
+    a.g.cell((unsigned)t, i, j, k);
     if (i < a.start.x || j < a.start.y || k < a.start.z || i >= a.end.x || j >= a.end.y ||
         k >= a.end.z)
         return;
@@ -118,6 +118,8 @@ launch_general_fill(AcReal* const* fields, int nfields, int nx, int ny, int nz, 
         a.f[q] = q < nfields ? fields[q] : nullptr;
     a.nfields = nfields;
     a.g       = GhostShell::of(nx, ny, nz);
+    if (!a.g.fits_32bit())
+        return hipErrorInvalidValue;
     a.start   = start;
     a.end     = end;
     for (int d = 0; d < 3; ++d) {
@@ -143,6 +145,8 @@ launch_periodic_fill(AcReal* const* fields, int nfields, int nx, int ny, int nz,
         a.f[q] = q < nfields ? fields[q] : nullptr;
     a.nfields = nfields;
     a.g       = GhostShell::of(nx, ny, nz);
+    if (!a.g.fits_32bit())
+        return hipErrorInvalidValue;
     a.start   = start;
     a.end     = end;
     const long total = a.g.cells();

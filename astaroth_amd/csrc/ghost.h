@@ -28,45 +28,62 @@ struct GhostShell {
     }
     __host__ __device__ long cells() const { return cz + cy + cx; }
 
-    // Ghost cell t (0 <= t < cells()) -> (i, j, k).
-    __device__ __forceinline__ void cell(long t, int& i, int& j, int& k) const
+    // Shells whose cell count fits 32 bits (every box up to about 10^4 cells per axis) are
+    // enumerated in 32-bit arithmetic: a 64-bit division costs several times a 32-bit one, and
+    // the fill does little else per cell.
+    __host__ __device__ bool fits_32bit() const { return cells() < 0xFFFFFFFFL; }
+
+    // Ghost cell t (0 <= t < cells() < 2^32) -> (i, j, k).
+    __device__ __forceinline__ void cell(unsigned t, int& i, int& j, int& k) const
     {
-        if (t < cz) {
-            const long plane = (long)mx * my;
-            const int pz     = (int)(t / plane);
-            const long rem   = t - (long)pz * plane;
-            k                = pz < 3 ? pz : nz + pz;
-            j                = (int)(rem / mx);
-            i                = (int)(rem - (long)j * mx);
+        const unsigned ucz = (unsigned)cz, ucy = (unsigned)cy, umx = (unsigned)mx;
+        if (t < ucz) {
+            const unsigned plane = umx * (unsigned)my;
+            const unsigned pz    = t / plane;
+            const unsigned rem   = t - pz * plane;
+            const unsigned jj    = rem / umx;
+            k                    = pz < 3 ? (int)pz : nz + (int)pz;
+            j                    = (int)jj;
+            i                    = (int)(rem - jj * umx);
         }
-        else if (t < cz + cy) {
-            const long u    = t - cz;
-            const long slab = 6L * mx;
-            const int kk    = (int)(u / slab);
-            const long rem  = u - (long)kk * slab;
-            const int jr    = (int)(rem / mx);
-            k               = 3 + kk;
-            j               = jr < 3 ? jr : ny + jr;
-            i               = (int)(rem - (long)jr * mx);
+        else if (t < ucz + ucy) {
+            const unsigned u    = t - ucz;
+            const unsigned slab = 6u * umx;
+            const unsigned kk   = u / slab;
+            const unsigned rem  = u - kk * slab;
+            const unsigned jr   = rem / umx;
+            k                   = 3 + (int)kk;
+            j                   = jr < 3 ? (int)jr : ny + (int)jr;
+            i                   = (int)(rem - jr * umx);
         }
         else {
-            const long u    = t - cz - cy;
-            const long slab = 6L * ny;
-            const int kk    = (int)(u / slab);
-            const int rem   = (int)(u - (long)kk * slab);
-            const int ir    = rem % 6;
-            k               = 3 + kk;
-            j               = 3 + rem / 6;
-            i               = ir < 3 ? ir : nx + ir;
+            const unsigned u    = t - ucz - ucy;
+            const unsigned slab = 6u * (unsigned)ny;
+            const unsigned kk   = u / slab;
+            const unsigned rem  = u - kk * slab;
+            const unsigned row  = rem / 6u;
+            const unsigned ir   = rem - 6u * row;
+            k                   = 3 + (int)kk;
+            j                   = 3 + (int)row;
+            i                   = ir < 3 ? (int)ir : nx + (int)ir;
         }
     }
 
-    // Linear indices of ghost cell (i, j, k) and of its periodic interior image.
+    // Interior image of coordinate c in [0, n + 6) on an axis of n interior cells: one shift by n
+    // when n >= 3, else as many as it takes.
+    static __device__ __forceinline__ int image(int c, int n)
+    {
+        if (n >= 3)
+            return c < 3 ? c + n : c >= n + 3 ? c - n : c;
+        const int r = (c - 3) % n;
+        return (r < 0 ? r + n : r) + 3;
+    }
+
+    // Linear indices of ghost cell (i, j, k) (each coordinate in [0, n + 6)) and of its periodic
+    // interior image.
     __device__ __forceinline__ void indices(int i, int j, int k, long& dst, long& src) const
     {
-        const int is = (i - 3 + nx) % nx + 3;
-        const int js = (j - 3 + ny) % ny + 3;
-        const int ks = (k - 3 + nz) % nz + 3;
+        const int is = image(i, nx), js = image(j, ny), ks = image(k, nz);
         const long mxy = (long)mx * my;
         dst            = i + (long)j * mx + (long)k * mxy;
         src            = is + (long)js * mx + (long)ks * mxy;

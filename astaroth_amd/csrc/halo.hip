@@ -44,11 +44,15 @@ pack_kernel(const PackArgs a)
     for (int q = 1; q < a.nblocks; ++q)
         r = t >= a.b[q].first ? q : r;
     const PackBlock& b = a.b[r];
-    const long local   = t - b.first;
-    const long row     = local / b.dims.x;
-    const int x        = (int)(local - row * b.dims.x);
-    const int y        = (int)(row % b.dims.y);
-    const int z        = (int)(row / b.dims.y);
+    // 32-bit arithmetic within a block (fewer than 2^32 cells, checked at launch): a 64-bit
+    // division costs several times a 32-bit one
+    const unsigned local = (unsigned)(t - b.first);
+    const unsigned dx = (unsigned)b.dims.x, dy = (unsigned)b.dims.y;
+    const unsigned row = local / dx;
+    const int x        = (int)(local - row * dx);
+    const unsigned zz  = row / dy;
+    const int y        = (int)(row - zz * dy);
+    const int z        = (int)zz;
     const long m = (long)(b.origin.x + x) + (long)(b.origin.y + y) * a.mx + (long)(b.origin.z + z) * a.mxy;
     AcReal* p    = a.packed + b.offset + local;
 #pragma unroll
@@ -81,6 +85,8 @@ launch_pack(bool pack, AcReal* const* fields, int mx, int my, AcReal* packed, in
         b.origin     = origins[i];
         b.dims       = dims[i];
         b.cells      = (long)dims[i].x * dims[i].y * dims[i].z;
+        if (b.cells >= 0xFFFFFFFFL)
+            return hipErrorInvalidValue;
         b.first      = first;
         b.offset     = offset;
         first += b.cells;
