@@ -49,7 +49,7 @@ periodic_fill_batch(const FillBox* __restrict__ boxes, const int nboxes, const l
     }
     const FillBox& b = boxes[lo];
     int i, j, k;
-    b.g.cell(t - b.first, i, j, k);
+    b.g.cell((unsigned)(t - b.first), i, j, k);
     long dst, src;
     b.g.indices(i, j, k, dst, src);
 #pragma unroll
@@ -483,6 +483,10 @@ acEnsembleCreate(const int device_id, const int nboxes, const AcMeshInfo* config
                 db[b].f[q] = e->fields[p][q][b];
             }
             fb[b].g     = GhostShell::of(n.x, n.y, n.z);
+            if (!fb[b].g.fits_32bit()) {
+                fprintf(stderr, "acEnsembleCreate: box %d is too large for the batched fill\n", b);
+                return AC_FAILURE;
+            }
             fb[b].first = first;
             first += fb[b].g.cells();
             db[b].nx  = n.x;

@@ -9,19 +9,19 @@
 //     src != dst.
 //   acNodeIntegrateSubstep: the region clipped to the interior, as node.cc:595-612 clips it to
 //     each subgrid, and skipped when empty.
-//   acNodeIntegrate / acNodeIntegrateGBC: per substep the interior is split, as node.cc:729-880
-//     splits it, into an inner region that reads no ghost cell and an outer region that does.
-//     The inner region runs on STREAM_1 while STREAM_0 fills the ghost shell and then solves the
-//     outer region, then the buffers swap. (The reference fills before it starts the inner
-//     region: what its inner region overlaps is the exchange between GPUs, which one GPU does
-//     not have.) The outer region is cut to suit the z-march's tiles (split_step below) rather
-//     than into the reference's six 3-cell slabs. The result is that of one launch over the
-//     interior to rounding: the z-march accumulates a cell's cross-derivative sums in the phase
-//     of its unrolled plane loop that the column's first plane sets, and the phases are compiled
-//     separately, so a few cells differ in their last bits. Boxes no more than two tiles wide in
-//     x or y, or 2 NGHOST deep in z, integrate the interior in one launch. Where the Device reads
-//     the ghost cells' periodic images (acDeviceSubstepReadsPeriodicImages), there is no fill to
-//     overlap and acNodeIntegrate runs each substep as one such launch.
+//   acNodeIntegrate / acNodeIntegrateGBC: the Device's RK3 step (acDeviceIntegrateStep /
+//     acDeviceIntegrateStepGBC on STREAM_0), between synchronisations of every stream. The
+//     reference splits each substep into an inner region, integrated while the halos are
+//     exchanged between GPUs, and six outer slabs (node.cc:729-880). On one GPU there is no
+//     exchange: the halo is the box's own periodic fill (or images read in its place), 4 % of a
+//     512^3 FP32 step, and every split measured slower than the step it would overlap (512^3
+//     FP32, same box, against 12.8 ms for the Device step): the reference's six 3-cell slabs
+//     after an inner region [2N, n) on another stream 17.0 ms (a 3-cell slab leaves 13 of a
+//     16-wide tile's lanes idle); a one-tile ring in x and y with the inner region over all of
+//     z (reading its z ghosts' images) 15.9 ms with the ring in 28-plane z ranges, 28.9 ms in
+//     whole columns queued piece after piece, 19.3 ms in whole columns in one launch (the GPU
+//     dispatches the inner launch's remaining blocks before the ring's, so the ring runs as a
+//     third round of columns). The result is the Device step's, bit for bit.
 //   acNodeReduce*: the Device reduction, finalised as node.cc:932-957 finalises the per-GPU
 //     results: RMS, RMS_EXP and ALFVEN_RMS return sqrt(sum / (n_x n_y n_z)).
 //   acNodePrintInfo, acNodeAutoOptimize: the Device's (the reference returns AC_FAILURE for both,
@@ -81,81 +81,6 @@ finalise(const Node node, ReductionType rtype, AcReal sum)
         return std::sqrt(inv_n * sum);
     }
     return sum;
-}
-
-// The step of node.cc:729-880 on one Device; `fill` fills the ghost shell of the `in` buffers
-// on the given stream.
-//
-// The reference's outer region is six slabs NGHOST cells thick. Solved by tiles of the z-march
-// (16 x 16 cells by default), a slab 3 cells thin in x or y leaves 13 of 16 lanes of every row
-// idle, and a 3-plane z slab marches 6 warm-up planes per 3 outputs: that made the split step
-// 34 % slower than one launch over the interior. Here the x-y part of the outer region is a ring
-// one tile wide (a_x = max(NGHOST, tile_x) cells, a_y likewise), solved over all of z, and only
-// the inner x-y part keeps the two z slabs; the inner region [N + a_x, n + N - a_x) x
-// [N + a_y, n + N - a_y) x [2N, n) still reads no ghost cell. The inner region runs on STREAM_1
-// while STREAM_0 fills the ghost shell and then, in stream order, solves the ring (4 boxes) and
-// the two z slabs, so no host wait separates the fill from the outer region.
-template <typename Fill>
-AcResult
-split_step(const Node node, const AcReal dt, Fill fill)
-{
-    const Device d = node->device;
-    const int3 n   = node->grid.n;
-    const int N    = NGHOST;
-    AcKernelConfig whole;
-    if (acDeviceGetKernelConfig(d, &whole) != AC_SUCCESS)
-        return AC_FAILURE;
-    const int ax = whole.tile_x > N ? whole.tile_x : N;
-    const int ay = whole.tile_y > N ? whole.tile_y : N;
-    const bool split = n.x > 2 * ax && n.y > 2 * ay && n.z > 2 * N;
-    // Ring columns span the whole depth in 28-plane z ranges, so that each ring launch has
-    // enough blocks to spread over the GPU beside the inner region (a whole-depth column per
-    // tile leaves a 512-cell side of the ring 32 blocks).
-    AcKernelConfig ring = whole;
-    if (ring.variant != 0)
-        ring.z_depth = 28;
-    if (acDeviceSynchronizeStream(d, STREAM_ALL) != AC_SUCCESS)
-        return AC_FAILURE;
-    for (int s = 0; s < 3; ++s) {
-        if (fill(STREAM_0) != AC_SUCCESS)
-            return AC_FAILURE;
-        if (!split) {
-            if (acDeviceIntegrateSubstep(d, STREAM_0, s, i3(N, N, N),
-                                         i3(n.x + N, n.y + N, n.z + N), dt) != AC_SUCCESS)
-                return AC_FAILURE;
-        }
-        else {
-            const int x0 = N + ax, x1 = n.x + N - ax, y0 = N + ay, y1 = n.y + N - ay;
-            // inner region: reads cells [N, n + N) only, concurrently with the fill
-            if (acDeviceIntegrateSubstep(d, STREAM_1, s, i3(x0, y0, 2 * N), i3(x1, y1, n.z), dt) !=
-                AC_SUCCESS)
-                return AC_FAILURE;
-            const struct {
-                bool ring;
-                int3 start, end;
-            } outer[6] = {
-                {true, i3(N, N, N), i3(n.x + N, y0, n.z + N)},        // ring, low y
-                {true, i3(N, y1, N), i3(n.x + N, n.y + N, n.z + N)},  // ring, high y
-                {true, i3(N, y0, N), i3(x0, y1, n.z + N)},            // ring, low x
-                {true, i3(x1, y0, N), i3(n.x + N, y1, n.z + N)},      // ring, high x
-                {false, i3(x0, y0, N), i3(x1, y1, 2 * N)},            // inner x-y, low z
-                {false, i3(x0, y0, n.z), i3(x1, y1, n.z + N)},        // inner x-y, high z
-            };
-            AcResult r = AC_SUCCESS;
-            for (const auto& o : outer) {
-                if (r == AC_SUCCESS)
-                    r = acDeviceSetKernelConfig(d, o.ring ? ring : whole);
-                if (r == AC_SUCCESS)
-                    r = acDeviceIntegrateSubstep(d, STREAM_0, s, o.start, o.end, dt);
-            }
-            if (acDeviceSetKernelConfig(d, whole) != AC_SUCCESS || r != AC_SUCCESS)
-                return AC_FAILURE;
-        }
-        if (acDeviceSynchronizeStream(d, STREAM_ALL) != AC_SUCCESS ||
-            acDeviceSwapBuffers(d) != AC_SUCCESS)
-            return AC_FAILURE;
-    }
-    return AC_SUCCESS;
 }
 
 } // namespace
@@ -389,21 +314,11 @@ acNodeIntegrate(const Node node, const AcReal dt)
 {
     if (!valid(node, "acNodeIntegrate"))
         return AC_FAILURE;
-    if (acDeviceSubstepReadsPeriodicImages(node->device)) {
-        // the Device's build reads the ghost cells' periodic images: no fill to wait for or to
-        // overlap, so each substep is one launch over the interior
-        if (acDeviceSynchronizeStream(node->device, STREAM_ALL) != AC_SUCCESS)
-            return AC_FAILURE;
-        for (int s = 0; s < 3; ++s)
-            if (acDeviceIntegrateSubstepPeriodic(node->device, STREAM_0, s, dt) != AC_SUCCESS ||
-                acDeviceSwapBuffers(node->device) != AC_SUCCESS)
-                return AC_FAILURE;
-        return acDeviceSynchronizeStream(node->device, STREAM_ALL);
-    }
-    const int3 m = node->grid.m;
-    return split_step(node, dt, [&](Stream s) {
-        return acDevicePeriodicBoundconds(node->device, s, i3(0, 0, 0), m);
-    });
+    const Device d = node->device;
+    if (acDeviceSynchronizeStream(d, STREAM_ALL) != AC_SUCCESS ||
+        acDeviceIntegrateStep(d, STREAM_0, dt) != AC_SUCCESS)
+        return AC_FAILURE;
+    return acDeviceSynchronizeStream(d, STREAM_ALL);
 }
 
 AcResult
@@ -411,11 +326,11 @@ acNodeIntegrateGBC(const Node node, const AcMeshInfo config, const AcReal dt)
 {
     if (!valid(node, "acNodeIntegrateGBC"))
         return AC_FAILURE;
-    const int3 m = node->grid.m;
-    return split_step(node, dt, [&](Stream s) {
-        return acDeviceGeneralBoundconds(node->device, s, i3(0, 0, 0), m, config,
-                                         i3(-1, -1, -1));
-    });
+    const Device d = node->device;
+    if (acDeviceSynchronizeStream(d, STREAM_ALL) != AC_SUCCESS ||
+        acDeviceIntegrateStepGBC(d, STREAM_0, config, dt) != AC_SUCCESS)
+        return AC_FAILURE;
+    return acDeviceSynchronizeStream(d, STREAM_ALL);
 }
 
 AcResult

@@ -506,8 +506,8 @@ def run_workload(a, kcfg, hip):
         line = dict(base, metric="cell-updates/s per RK3 step of acNodeIntegrate at %d^3 FP32" % a.n,
                     value=a.n ** 3 / (r["node_ms_per_step"] * 1e-3), unit="cell-updates/s", steps=5, warmup=1,
                     ms_per_step=r["node_ms_per_step"], dtype="f32",
-                    config={"workload": "8-field MHD, %d^3 periodic box, Node step (inner region during the fill, "
-                                        "then the outer region) against the Device step" % a.n}, results=r)
+                    config={"workload": "8-field MHD, %d^3 periodic box, Node step (the Device step between "
+                                        "synchronisations) against the Device step" % a.n}, results=r)
         print(json.dumps(line), flush=True)
         return 0
     if a.workload == "boundaries":
@@ -629,7 +629,7 @@ def reductions_benchmark(n: int, hip: Hip, reps=5):
 
 
 def node_benchmark(n: int, dt: float, steps=5, rounds=3):
-    """acNodeIntegrate (inner region during the ghost fill, then the outer region) against
+    """acNodeIntegrate (the Device step between synchronisations) against
     acDeviceIntegrateStep on the same n^3 FP32 state, alternating `rounds` times between the two
     so that both see the same clocks; host wall time per RK3 step (both are synchronous calls)."""
     from astaroth_amd.api import Device, Library

@@ -347,15 +347,10 @@ Node acGetNode(void);
  * - the *WithOffset load/store forms copy from `src` to `dst` as the Device forms do (the
  *   reference Node uses `src` for both, node.cc:452-455, 549-552);
  * - acNodeIntegrateSubstep clips the region to the interior and does nothing when it is empty;
- * - acNodeIntegrate / acNodeIntegrateGBC: per substep the inner region (x and y one kernel tile
- *   in from the faces, z in [2 NGHOST, n)), which reads no ghost cell, is integrated on STREAM_1
- *   while STREAM_0 fills the ghost shell (periodic / by the face types of `config`) and then
- *   integrates the outer region (a ring one tile wide in x and y over all of z, and the two
- *   NGHOST-deep z slabs of the inner x-y part), then the buffers swap; the result is that of
- *   3 x {fill, substep over the interior, swap} to rounding (the launches start their z-march
- *   columns at other planes); synchronous. Where the Device's periodic step
- *   reads the ghost cells' images (acDeviceSubstepReadsPeriodicImages), acNodeIntegrate is that
- *   step instead;
+ * - acNodeIntegrate / acNodeIntegrateGBC: the Device's RK3 step (acDeviceIntegrateStep /
+ *   acDeviceIntegrateStepGBC on STREAM_0), synchronous. The reference splits each substep into
+ *   an inner region integrated during the halo exchange between GPUs and six outer slabs; one
+ *   GPU has no exchange to hide, and the split measured slower than the step;
  * - acNodeReduce* finalise RMS, RMS_EXP and ALFVEN_RMS as sqrt(sum / n_x n_y n_z);
  * - acNodePrintInfo and acNodeAutoOptimize are the Device's (the reference returns AC_FAILURE).
  * The load, store, set and reduction calls first synchronise their stream, as the reference

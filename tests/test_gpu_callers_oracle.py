@@ -2,8 +2,7 @@
 5 RK3 steps), as the reference verifies its callers' steps against the host model
 (samples/mpitest/main.cc:357-363, samples/ctest/main.c:36-55):
 
-* acNodeIntegrate: inner region during the ghost fill, then the outer region (node.cc:729-806);
-  its one-tile ring spans several 28-plane z ranges here;
+* acNodeIntegrate (the Device step on one GPU; reference node.cc:729-806);
 * the legacy acIntegrate (astaroth.cc:28-188);
 * acEnsembleIntegrateStepGraph: the box inside a three-box ensemble (persistent batched launch,
   periodic images, one graph per step);
