@@ -1511,7 +1511,13 @@ zlean_column(const SubstepArgs& a,
     k.template step<4, false>(st, pbeg + 4);
     k.template step<5, false>(st, pbeg + 5);
     if (NQ == 7) {
-        for (int p = pbeg + 6; p < k.ze + 3; p += 7) {
+        // Output planes z = p - 3 for p in [pbeg + 6, ze + 3): whole rounds of the 7 phases,
+        // then the phases of a last, partial round up to the last output plane, so that a column
+        // whose depth is not a multiple of 7 computes no plane past its end (the branches are
+        // uniform; the partial round is a second copy of the phases' code, outside the loop).
+        const int pend = k.ze + 3;
+        int p          = pbeg + 6;
+        for (; p + 7 <= pend; p += 7) {
             k.template step<6, true>(st, p);
             k.template step<0, true>(st, p + 1);
             k.template step<1, true>(st, p + 2);
@@ -1519,6 +1525,23 @@ zlean_column(const SubstepArgs& a,
             k.template step<3, true>(st, p + 4);
             k.template step<4, true>(st, p + 5);
             k.template step<5, true>(st, p + 6);
+        }
+        if (p < pend) {
+            k.template step<6, true>(st, p);
+            if (p + 1 < pend) {
+                k.template step<0, true>(st, p + 1);
+                if (p + 2 < pend) {
+                    k.template step<1, true>(st, p + 2);
+                    if (p + 3 < pend) {
+                        k.template step<2, true>(st, p + 3);
+                        if (p + 4 < pend) {
+                            k.template step<3, true>(st, p + 4);
+                            if (p + 5 < pend)
+                                k.template step<4, true>(st, p + 5);
+                        }
+                    }
+                }
+            }
         }
     }
     else {

@@ -11,7 +11,8 @@ side than the one before, and the block's outer cells stand in for the ghost cel
 
 Bars (range-normalised over the whole box, at the slice's worst cell):
 * FP64 library: <= 1e-12 (rounding of 30 substeps in FP64);
-* FP32 library: <= 1e-5 (the precision study's target).
+* FP32 library: <= 1e-5 (the precision study's target), plain and with compensated
+  integration (acDeviceSetCompensatedIntegration: FP32 state plus its rounding-error term).
 """
 import ctypes
 import sys
@@ -49,10 +50,12 @@ def planes(lib, dev, info, z0, nz):
     return [f[z0:z0 + nz].copy() for f in mesh.fields]
 
 
-def run_library(double):
+def run_library(double, compensated=False):
     lib = Library(double)
     info = lib.config(n=N)
     with lib.device_create(0, info, autotune=False) as dev:
+        if compensated:
+            dev.set_compensated_integration(True)
         device_init_mhd(dev, config_id=3, perturb_sin2z=True)
         block = [p[:, B0:B0 + BW, B0:B0 + BW] for p in planes(lib, dev, info, B0, BW)] if double else None
         for _ in range(STEPS):
@@ -67,7 +70,10 @@ def run_library(double):
 @pytest.fixture(scope="module")
 def results():
     info, block, sl64, rng = run_library(True)
+    print("FP64 library done", flush=True)
     _, _, sl32, _ = run_library(False)
+    _, _, sl32c, _ = run_library(False, compensated=True)
+    print("FP32 libraries done, long double oracle on the block", flush=True)
     # long double on the block: 30 substeps over shrinking regions
     pyoracle.set_threads(16)
     a = [np.ascontiguousarray(f, dtype=np.longdouble) for f in block]
@@ -81,7 +87,7 @@ def results():
         pyoracle.substep_gpu_form(a, b, rp, j % 3, DT, start, end)
         a, b = b, a
     ref = [f[lo:lo + SW, lo:lo + SW, lo:lo + SW].astype(np.float64) for f in a]
-    return ref, sl64, sl32, rng
+    return ref, sl64, sl32, sl32c, rng
 
 
 def errors(ref, got, rng):
@@ -93,7 +99,7 @@ def errors(ref, got, rng):
 
 
 def test_fp64_slice_against_long_double(results):
-    ref, sl64, _, rng = results
+    ref, sl64, _, _, rng = results
     err = errors(ref, sl64, rng)
     print("FP64 vs long double, range-normalised:", {k: "%.2e" % v for k, v in err.items()})
     assert all(np.isfinite(v) for v in err.values())
@@ -101,7 +107,15 @@ def test_fp64_slice_against_long_double(results):
 
 
 def test_fp32_slice_against_long_double(results):
-    ref, _, sl32, rng = results
+    ref, _, sl32, _, rng = results
     err = errors(ref, sl32, rng)
     print("FP32 vs long double, range-normalised:", {k: "%.2e" % v for k, v in err.items()})
+    assert max(err.values()) <= 1e-5, err
+
+
+def test_fp32_compensated_slice_against_long_double(results):
+    ref, _, _, sl32c, rng = results
+    err = errors(ref, sl32c, rng)
+    print("FP32 compensated vs long double, range-normalised:", {k: "%.2e" % v for k, v in err.items()})
+    assert all(np.isfinite(v) for v in err.values())
     assert max(err.values()) <= 1e-5, err
