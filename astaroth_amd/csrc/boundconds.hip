@@ -30,9 +30,12 @@ struct FillArgs {
     int3 start, end;
 };
 
+// NFIX: the field count when fixed at compile time (all eight fields, the common call), else 0
+template <int NFIX>
 __global__ void __launch_bounds__(256)
 periodic_fill_kernel(const FillArgs a)
 {
+    const int nf = NFIX ? NFIX : a.nfields;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.g.cells())
         return;
@@ -43,10 +46,18 @@ periodic_fill_kernel(const FillArgs a)
         return;
     long dst, src;
     a.g.indices(i, j, k, dst, src);
+    // Every field's load is issued before the first store. The field pointers may alias as far
+    // as the compiler knows, so field by field each load would wait for the store before it:
+    // eight memory round trips in a row per thread instead of one.
+    AcReal v[MAX_FILL_FIELDS];
 #pragma unroll
     for (int q = 0; q < MAX_FILL_FIELDS; ++q)
-        if (q < a.nfields)
-            a.f[q][dst] = a.f[q][src];
+        if (q < nf)
+            v[q] = a.f[q][src];
+#pragma unroll
+    for (int q = 0; q < MAX_FILL_FIELDS; ++q)
+        if (q < nf)
+            a.f[q][dst] = v[q];
 }
 
 // Per axis and side: AC_BOUNDCOND_PERIODIC (0), _SYMMETRIC (1) or _ANTISYMMETRIC (2).
@@ -79,9 +90,11 @@ bc_source(int c, int n, int bot, int top, bool& negate)
     return c;
 }
 
+template <int NFIX>
 __global__ void __launch_bounds__(256)
 general_fill_kernel(const GeneralArgs a)
 {
+    const int nf = NFIX ? NFIX : a.nfields;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.g.cells())
         return;
@@ -97,12 +110,15 @@ general_fill_kernel(const GeneralArgs a)
     const long mxy = (long)a.g.mx * a.g.my;
     const long dst = i + (long)j * a.g.mx + (long)k * mxy;
     const long src = is + (long)js * a.g.mx + (long)ks * mxy;
+    AcReal v[MAX_FILL_FIELDS]; // all loads before the first store, as in the periodic fill
 #pragma unroll
     for (int q = 0; q < MAX_FILL_FIELDS; ++q)
-        if (q < a.nfields) {
-            const AcReal v = a.f[q][src];
-            a.f[q][dst]    = negate ? -v : v;
-        }
+        if (q < nf)
+            v[q] = a.f[q][src];
+#pragma unroll
+    for (int q = 0; q < MAX_FILL_FIELDS; ++q)
+        if (q < nf)
+            a.f[q][dst] = negate ? -v[q] : v[q];
 }
 
 } // namespace
@@ -130,7 +146,10 @@ launch_general_fill(AcReal* const* fields, int nfields, int nx, int ny, int nz, 
     const long blocks = (total + 255) / 256;
     if (blocks > 0x7fffffffL)
         return hipErrorInvalidConfiguration;
-    general_fill_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(a);
+    if (nfields == MAX_FILL_FIELDS)
+        general_fill_kernel<MAX_FILL_FIELDS><<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(a);
+    else
+        general_fill_kernel<0><<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(a);
     return hipGetLastError();
 }
 
@@ -153,7 +172,10 @@ launch_periodic_fill(AcReal* const* fields, int nfields, int nx, int ny, int nz,
     const long blocks = (total + 255) / 256;
     if (blocks > 0x7fffffffL)
         return hipErrorInvalidConfiguration;
-    periodic_fill_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(a);
+    if (nfields == MAX_FILL_FIELDS)
+        periodic_fill_kernel<MAX_FILL_FIELDS><<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(a);
+    else
+        periodic_fill_kernel<0><<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(a);
     return hipGetLastError();
 }
 

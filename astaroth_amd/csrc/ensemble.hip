@@ -52,9 +52,15 @@ periodic_fill_batch(const FillBox* __restrict__ boxes, const int nboxes, const l
     b.g.cell((unsigned)(t - b.first), i, j, k);
     long dst, src;
     b.g.indices(i, j, k, dst, src);
+    // all eight loads in flight before the first store (the field pointers may alias as far as
+    // the compiler knows, which would otherwise make each load wait for the store before it)
+    AcReal v[NF];
 #pragma unroll
     for (int q = 0; q < NF; ++q)
-        b.f[q][dst] = b.f[q][src];
+        v[q] = b.f[q][src];
+#pragma unroll
+    for (int q = 0; q < NF; ++q)
+        b.f[q][dst] = v[q];
 }
 
 struct DiagBox {

@@ -55,12 +55,19 @@ pack_kernel(const PackArgs a)
     const int z        = (int)zz;
     const long m = (long)(b.origin.x + x) + (long)(b.origin.y + y) * a.mx + (long)(b.origin.z + z) * a.mxy;
     AcReal* p    = a.packed + b.offset + local;
+    // All eight loads are issued before the first store: the buffer and the fields may alias
+    // as far as the compiler knows, and in program order field by field each load would wait
+    // for the store before it.
+    AcReal v[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+        v[f] = PACK ? a.f[f][m] : p[f * b.cells];
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
         if (PACK)
-            p[f * b.cells] = a.f[f][m];
+            p[f * b.cells] = v[f];
         else
-            a.f[f][m] = p[f * b.cells];
+            a.f[f][m] = v[f];
     }
 }
 
