@@ -111,6 +111,9 @@ hipError_t launch_substep_batch(int tile_x, int tile_y, const SubstepArgs* boxes
 int kernel_config_count();
 AcKernelConfig kernel_config_at(int index);
 AcKernelConfig default_kernel_config(int nx, int ny, int nz);
+// z-depth of equal z-chunks for a z-march of tile_x x tile_y columns with `blocks_per_cu` blocks
+// resident per CU (0: whole columns), chosen by the launch's rounds of resident blocks.
+int balanced_z_depth(int nx, int ny, int nz, int tile_x, int tile_y, int blocks_per_cu);
 
 // Periodic fill of the ghost cells inside [start, end) of `nfields` buffers (boundconds.hip).
 hipError_t launch_periodic_fill(AcReal* const* fields, int nfields, int nx, int ny, int nz,

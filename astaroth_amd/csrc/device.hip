@@ -1311,10 +1311,12 @@ acDeviceAutoOptimize(const Device device)
     // untimed launches first, so that the first configuration timed does not pay for the clocks
     // ramping up from idle
     (void)time_config(device, best, 4 * iters);
-    // z-depths: powers of two for variant 1, whole rounds of the unrolled plane loop (7 planes,
-    // 8 for variant 13) for the other marches
+    // z-depths: powers of two for variant 1, whole rounds of the unrolled 8-plane loop for
+    // variant 13, and for the other lean marches (whose plane loop ends with a partial round)
+    // equal chunks: the box's depth cut in 2, 3, 4 or 8
+    const int nz        = nz_of(device);
     const int depths1[] = {8, 16, 32, 64, 128, 0};
-    const int depths7[] = {14, 28, 56, 112, 0};
+    const int depths7[] = {(nz + 1) / 2, (nz + 2) / 3, (nz + 3) / 4, (nz + 7) / 8, 0};
     const int depths8[] = {16, 32, 64, 128, 0};
     for (int i = 0; i < kernel_config_count(); ++i) {
         AcKernelConfig c = kernel_config_at(i);

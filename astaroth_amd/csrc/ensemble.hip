@@ -6,7 +6,8 @@
 // One RK3 substep is
 //   * one periodic-fill launch over the ghost shells of all boxes (a flat list of ghost cells),
 //   * one persistent solve launch (solve_zlean_batch) over a (box, tile column, z range) work
-//     list sorted by decreasing cost and dealt round-robin to the resident blocks,
+//     list sorted by decreasing cost, dealt to the resident blocks in rounds of one item per
+//     block, each XCD taking a contiguous range of a round's items,
 // and a step ends with per-box FP64 diagnostics (max and sum of squares of |u| and |A|, max, min
 // and sum of squares of lnrho) in two launches. acEnsembleIntegrateStepGraph records the whole
 // step once per (dt, buffer parity) into a HIP graph and replays it.
@@ -237,8 +238,8 @@ build_items(Ensemble e)
         for (int tz = 0; tz < ntz; ++tz) {
             const int zs    = NGHOST + tz * zd;
             const int depth = std::min(zd, n.z + NGHOST - zs);
-            // planes marched: 6 warm-up planes plus the outputs rounded up to 7-plane rounds
-            const long cost = 6 + ((depth + 6) / 7) * 7;
+            // planes marched: 6 warm-up planes plus the outputs
+            const long cost = 6 + depth;
             for (int ty = 0; ty < nty; ++ty)
                 for (int tx = 0; tx < ntx; ++tx)
                     items.push_back({{b, tx, ty, zs, zd}, cost});
@@ -621,9 +622,9 @@ acEnsembleAutoOptimize(Ensemble e)
     if (build_items(e) != AC_SUCCESS)
         return AC_FAILURE;
     // 2. Per size class (the column length n_z), the class with the most cells first: whole
-    // columns, or columns cut into 2 or 4 z ranges of whole 7-plane rounds. A cut shortens the
-    // longest work items (the launch's tail) at the cost of 6 warm-up planes per extra item; it
-    // is kept only when the whole launch gets at least 2 % faster.
+    // columns, or columns cut into 2, 3 or 4 equal z ranges. A cut shortens the longest work
+    // items (the launch's tail) at the cost of 6 warm-up planes per extra item; it is kept only
+    // when the whole launch gets at least 2 % faster.
     std::map<int, double> work;
     for (const AcMeshInfo& i : e->info) {
         const int3 n = dims_of(i);
@@ -637,9 +638,9 @@ acEnsembleAutoOptimize(Ensemble e)
     for (const auto& c : classes) {
         const int nz = c.second;
         int keep     = 0;
-        for (const int parts : {2, 4}) {
-            const int d = ((nz + parts - 1) / parts + 6) / 7 * 7;
-            if (d < 14 || d >= nz)
+        for (const int parts : {2, 3, 4}) {
+            const int d = (nz + parts - 1) / parts;
+            if (d < 8 || d >= nz)
                 continue;
             e->class_depth[nz] = d;
             if (build_items(e) != AC_SUCCESS)

@@ -42,6 +42,9 @@
 //  the whole-step entry points of a periodic box where they are the faster form.
 #include "physics.h"
 
+#include <algorithm>
+#include <cmath>
+
 namespace ac {
 
 namespace {
@@ -1591,7 +1594,11 @@ solve_zlean_batch(const SubstepArgs* __restrict__ boxes, const BatchItem* __rest
     // periodic images read directly: the ensemble step fills no ghosts between its substeps
     __shared__
         typename ZMarchLean<TX, TY, false, NSLOT, DEFER, PREV_FIRST, 7, WIN, true>::Shared sh;
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    // Round r takes items [r G, (r + 1) G) (G = gridDim.x), each XCD a contiguous range of them
+    // (xcd_remap): consecutive items are mostly neighbouring tile columns of one box, whose halo
+    // lines are then shared in one L2.
+    const int slot = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    for (int it = slot; it < nitems; it += gridDim.x) {
         const BatchItem w = items[it];
         zlean_column<TX, TY, false, NSLOT, DEFER, PREV_FIRST, 7, WIN, true>(
             boxes[w.box], sh, w.btx, w.bty, w.zs, w.zdepth);
@@ -1745,6 +1752,49 @@ kernel_config_at(int index)
     return c;
 }
 
+// Compute units of the current GPU (read once; 256 if the query fails).
+static int
+current_cu_count()
+{
+    static const int cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        return n;
+    }();
+    return cus;
+}
+
+int
+balanced_z_depth(int nx, int ny, int nz, int tile_x, int tile_y, int blocks_per_cu)
+{
+    // A z-march block costs its depth plus the 6 planes that fill its queues before the first
+    // output. Blocks are dispatched in rounds of `slots` (the resident block slots of the GPU),
+    // so a launch costs about rounds x (depth + 6). Splits of the box into k equal z-chunks of at
+    // least 8 planes are considered up to four times the k that fills every slot in the first
+    // round, and the cheapest is taken (the smallest k among equals). Equal chunks keep each
+    // XCD's contiguous block range equally loaded: an uneven last chunk leaves the XCDs that hold
+    // its blocks idle for the rest of the launch.
+    const long cols  = (long)((nx + tile_x - 1) / tile_x) * ((ny + tile_y - 1) / tile_y);
+    const long slots = (long)blocks_per_cu * current_cu_count();
+    const long kfill = std::max(1L, (slots + cols - 1) / cols);
+    const int kmax   = (int)std::max(1L, std::min(4 * kfill, (long)nz / 8));
+    int best_zd      = nz;
+    double best_cost = INFINITY;
+    for (int k = 1; k <= kmax; ++k) {
+        const int zd       = (nz + k - 1) / k;
+        const long blocks  = cols * ((nz + zd - 1) / zd);
+        const long rounds  = (blocks + slots - 1) / slots;
+        const double cost  = (double)rounds * (zd + 6);
+        if (cost < best_cost) {
+            best_cost = cost;
+            best_zd   = zd;
+        }
+    }
+    return best_zd >= nz ? 0 : best_zd;
+}
+
 AcKernelConfig
 default_kernel_config(int nx, int ny, int nz)
 {
@@ -1752,29 +1802,13 @@ default_kernel_config(int nx, int ny, int nz)
     // previous stage first and deferring its stores; in FP32 compiled for two waves per SIMD with
     // pinned LDS windows (variant 19: two blocks of 4 waves per CU, 64 KiB of LDS each), in FP64
     // for one wave per SIMD (variant 8: its 128 KiB ring leaves room for one block per CU). The
-    // z extent of a block is cut so that the grid still has about four blocks per concurrently
-    // resident block slot, in multiples of the unrolled 7-plane loop.
+    // z extent of a block is cut into equal chunks when the tile columns alone leave resident
+    // block slots empty (balanced_z_depth).
     AcKernelConfig c;
-#if AC_DOUBLE_PRECISION == 0
-    c.variant        = 19;
-    c.tile_x         = 16;
-    c.tile_y         = 16;
-    const long want  = 1024;
-    const int period = 7;
-#else
-    c.variant        = 8;
-    c.tile_x         = 16;
-    c.tile_y         = 16;
-    const long want  = 1024;
-    const int period = 7;
-#endif
-    const long cols = (long)((nx + c.tile_x - 1) / c.tile_x) * ((ny + c.tile_y - 1) / c.tile_y);
-    const long nzb  = cols >= want ? 1 : (want + cols - 1) / cols;
-    int zd          = (int)((nz + nzb - 1) / nzb);
-    zd              = ((zd + period - 1) / period) * period;
-    if (zd < 2 * period)
-        zd = 2 * period;
-    c.z_depth = zd >= nz ? 0 : zd;
+    c.variant = AC_DOUBLE_PRECISION ? 8 : 19;
+    c.tile_x  = 16;
+    c.tile_y  = 16;
+    c.z_depth = balanced_z_depth(nx, ny, nz, c.tile_x, c.tile_y, AC_DOUBLE_PRECISION ? 1 : 2);
     return c;
 }
 
