@@ -26,31 +26,50 @@ using namespace ac;
 namespace {
 
 constexpr int NDIAG     = 7;  // per-box diagnostics, see acEnsembleDiagnostics
-constexpr int DIAG_BLKS = 16; // blocks per box of the diagnostics launch
+constexpr int DIAG_BLKS = 64; // blocks per box of the diagnostics launch
+constexpr int DIAG_UNROLL = 4; // cells per lane whose loads are in flight together
+constexpr int FILL_BLOCK  = 256; // threads per block of the batched fill
+
+// The field pointers of a box come from a device array, so the compiler cannot tell their
+// address space and would use flat loads and stores; they are global memory.
+typedef __attribute__((address_space(1))) AcReal GlobalReal;
+__device__ __forceinline__ GlobalReal*
+global(AcReal* p)
+{
+    return (GlobalReal*)p;
+}
+__device__ __forceinline__ const GlobalReal*
+global(const AcReal* p)
+{
+    return (const GlobalReal*)p;
+}
 
 struct FillBox {
     AcReal* f[NF];
     GhostShell g;
-    long first; // index of the box's first ghost cell among all boxes
+    long first; // thread index of the box's first ghost cell, a multiple of FILL_BLOCK
 };
 
-__global__ void __launch_bounds__(256)
-periodic_fill_batch(const FillBox* __restrict__ boxes, const int nboxes, const long total)
+// Each box's ghost cells start on a block boundary, so a block serves one box and finds it with
+// a search keyed by its block index alone (uniform, scalar loads).
+__global__ void __launch_bounds__(FILL_BLOCK)
+periodic_fill_batch(const FillBox* __restrict__ boxes, const int nboxes)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total)
-        return;
-    int lo = 0, hi = nboxes - 1; // last box whose first cell is <= t
+    const long t0 = (long)blockIdx.x * FILL_BLOCK;
+    int lo = 0, hi = nboxes - 1; // last box whose first cell is <= t0
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (boxes[mid].first <= t)
+        if (boxes[mid].first <= t0)
             lo = mid;
         else
             hi = mid - 1;
     }
     const FillBox& b = boxes[lo];
+    const long u     = t0 - b.first + threadIdx.x;
+    if (u >= b.g.cells()) // the padding up to the next box's first block
+        return;
     int i, j, k;
-    b.g.cell((unsigned)(t - b.first), i, j, k);
+    b.g.cell((unsigned)u, i, j, k);
     long dst, src;
     b.g.indices(i, j, k, dst, src);
     // all eight loads in flight before the first store (the field pointers may alias as far as
@@ -58,10 +77,10 @@ periodic_fill_batch(const FillBox* __restrict__ boxes, const int nboxes, const l
     AcReal v[NF];
 #pragma unroll
     for (int q = 0; q < NF; ++q)
-        v[q] = b.f[q][src];
+        v[q] = global(b.f[q])[src];
 #pragma unroll
     for (int q = 0; q < NF; ++q)
-        b.f[q][dst] = v[q];
+        global(b.f[q])[dst] = v[q];
 }
 
 struct DiagBox {
@@ -89,6 +108,9 @@ is_max(int q)
     return q == 0 || q == 2 || q == 4;
 }
 
+// The interior of box blockIdx.y walked as one flat list of cells (x fastest, rows back to
+// back), so short rows leave no lane idle; each lane has the loads of DIAG_UNROLL cells in flight
+// before it accumulates them.
 __global__ void __launch_bounds__(256)
 diagnostics_batch(const DiagBox* __restrict__ boxes, double* __restrict__ partials)
 {
@@ -96,15 +118,33 @@ diagnostics_batch(const DiagBox* __restrict__ boxes, double* __restrict__ partia
     const DiagBox& b = boxes[blockIdx.y];
     double v[NDIAG]  = {0.0, 0.0, 0.0, 0.0, -INFINITY, INFINITY, 0.0};
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nrows = b.ny * b.nz;
-    for (int r = blockIdx.x * 4 + wave; r < nrows; r += gridDim.x * 4) {
-        const int j = 3 + r % b.ny, k = 3 + r / b.ny;
-        const long row = (long)j * b.mx + (long)k * b.mxy + 3;
-        for (int i = lane; i < b.nx; i += 64) {
-            const long c  = row + i;
-            const double ux = b.f[VTXBUF_UUX][c], uy = b.f[VTXBUF_UUY][c], uz = b.f[VTXBUF_UUZ][c];
-            const double ax = b.f[VTXBUF_AX][c], ay = b.f[VTXBUF_AY][c], az = b.f[VTXBUF_AZ][c];
-            const double lr = b.f[VTXBUF_LNRHO][c];
+    const unsigned nx = (unsigned)b.nx, nxy = (unsigned)b.nx * (unsigned)b.ny;
+    const unsigned ncell  = nxy * (unsigned)b.nz;
+    const unsigned stride = gridDim.x * 256u;
+    for (unsigned c0 = blockIdx.x * 256u + threadIdx.x; c0 < ncell; c0 += DIAG_UNROLL * stride) {
+        AcReal f[DIAG_UNROLL][7];
+        bool ok[DIAG_UNROLL];
+#pragma unroll
+        for (int uu = 0; uu < DIAG_UNROLL; ++uu) {
+            const unsigned c = c0 + uu * stride;
+            ok[uu]           = c < ncell;
+            const unsigned k = (ok[uu] ? c : 0u) / nxy, rem = (ok[uu] ? c : 0u) - k * nxy;
+            const unsigned j = rem / nx, i = rem - j * nx;
+            const long m = (long)(i + 3) + (long)(j + 3) * b.mx + (long)(k + 3) * b.mxy;
+            f[uu][0] = global(b.f[VTXBUF_UUX])[m];
+            f[uu][1] = global(b.f[VTXBUF_UUY])[m];
+            f[uu][2] = global(b.f[VTXBUF_UUZ])[m];
+            f[uu][3] = global(b.f[VTXBUF_AX])[m];
+            f[uu][4] = global(b.f[VTXBUF_AY])[m];
+            f[uu][5] = global(b.f[VTXBUF_AZ])[m];
+            f[uu][6] = global(b.f[VTXBUF_LNRHO])[m];
+        }
+#pragma unroll
+        for (int uu = 0; uu < DIAG_UNROLL; ++uu) {
+            if (!ok[uu])
+                continue;
+            const double ux = f[uu][0], uy = f[uu][1], uz = f[uu][2];
+            const double ax = f[uu][3], ay = f[uu][4], az = f[uu][5], lr = f[uu][6];
             const double u2 = ux * ux + uy * uy + uz * uz, a2 = ax * ax + ay * ay + az * az;
             v[0] = dmax(v[0], u2);
             v[1] += u2;
@@ -317,10 +357,11 @@ enqueue_fill(Ensemble e, int par)
 {
     if (e->fill_total == 0)
         return hipSuccess;
-    const long blocks = (e->fill_total + 255) / 256;
-    periodic_fill_batch<<<dim3((unsigned)blocks), dim3(256), 0, e->stream>>>(e->d_fill[par],
-                                                                              e->nboxes,
-                                                                              e->fill_total);
+    const long blocks = e->fill_total / FILL_BLOCK;
+    if (blocks > 0x7fffffffL)
+        return hipErrorInvalidConfiguration;
+    periodic_fill_batch<<<dim3((unsigned)blocks), dim3(FILL_BLOCK), 0, e->stream>>>(e->d_fill[par],
+                                                                                    e->nboxes);
     return hipGetLastError();
 }
 
@@ -494,8 +535,12 @@ acEnsembleCreate(const int device_id, const int nboxes, const AcMeshInfo* config
                 fprintf(stderr, "acEnsembleCreate: box %d is too large for the batched fill\n", b);
                 return AC_FAILURE;
             }
+            if ((long)n.x * n.y * n.z >= (1L << 31)) { // the diagnostics' 32-bit cell index
+                fprintf(stderr, "acEnsembleCreate: box %d is too large for the batched diagnostics\n", b);
+                return AC_FAILURE;
+            }
             fb[b].first = first;
-            first += fb[b].g.cells();
+            first += (fb[b].g.cells() + FILL_BLOCK - 1) / FILL_BLOCK * FILL_BLOCK;
             db[b].nx  = n.x;
             db[b].ny  = n.y;
             db[b].nz  = n.z;

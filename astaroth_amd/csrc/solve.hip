@@ -673,7 +673,7 @@ struct ZMarchLean {
             const unsigned ow      = pin(own);
 #pragma unroll
             for (int f = 0; f < NF; ++f)
-                *(AcReal*)((char*)(a.out[f] + oz) + ow) = st.res[f];
+                st_global(a.out[f] + oz, ow, st.res[f]);
         }
         st.res_z = -1;
     }
@@ -694,9 +694,21 @@ struct ZMarchLean {
         return off;
     }
 
+    // Field memory is global memory. The batched solve reads its pointers from a device array,
+    // whose address space the compiler cannot tell: without the cast its loads and stores are
+    // flat operations, which also count in the LDS counter, so every wait for a ring read would
+    // also wait for the next plane's loads in flight.
+    typedef __attribute__((address_space(1))) AcReal GReal;
+    typedef __attribute__((address_space(1))) const char GByte;
+
     static __device__ __forceinline__ AcReal ld(const AcReal* base, unsigned byte_off)
     {
-        return *(const AcReal*)((const char*)base + byte_off);
+        return *(const GReal*)((GByte*)base + byte_off);
+    }
+
+    static __device__ __forceinline__ void st_global(AcReal* base, unsigned byte_off, AcReal v)
+    {
+        *(GReal*)((GByte*)base + byte_off) = v;
     }
 
     // Ring slot of plane q (q >= 0).
@@ -1044,8 +1056,8 @@ struct ZMarchLean {
             const AcReal sum = u + d;
             const AcReal bb  = sum - u;
             const AcReal err = (u - (sum - bb)) + (d - bb);
-            *(AcReal*)((char*)(a.out[f] + oz) + ow)  = sum;
-            *(AcReal*)((char*)(a.cout[f] + oz) + ow) = err;
+            st_global(a.out[f] + oz, ow, sum);
+            st_global(a.cout[f] + oz, ow, err);
         }
     }
 };
@@ -1582,9 +1594,10 @@ solve_zlean(const SubstepArgs a, const int zdepth, const int ntx, const int nty)
 // Persistent form for ensembles of independent boxes: a grid sized to the resident block slots
 // walks a (box, tile column, z range) work list; each box brings its own arguments (pointers,
 // sizes, physical parameters), read from global memory.
-// The 16 x 16 FP32 build uses a 5-slot ring, previous stage first, deferred stores and two
-// waves per SIMD: on the ensemble's short columns (32-128 planes) it measured 11.4 ms per
-// 96-box step against 11.8 ms for the single-box default's options (pinned LDS windows).
+// The 16 x 16 FP32 build has the single-box default's options (4-slot ring, previous stage
+// first, deferred stores, pinned LDS windows, two waves per SIMD): 7.93 ms per 96-box step
+// against 8.87 ms for a 5-slot ring without pinned windows, which spills 20 registers to
+// scratch (interleaved on one box, work items dealt to XCDs in contiguous ranges).
 template <int TX, int TY, int W = 1, int NSLOT = 4, bool DEFER = false, bool PREV_FIRST = false,
           bool WIN = false>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(W)))
@@ -1606,7 +1619,7 @@ solve_zlean_batch(const SubstepArgs* __restrict__ boxes, const BatchItem* __rest
 }
 
 #if AC_DOUBLE_PRECISION == 0
-#define AC_BATCH16 solve_zlean_batch<16, 16, 2, 5, true, true>
+#define AC_BATCH16 solve_zlean_batch<16, 16, 2, 4, true, true, true>
 #else
 #define AC_BATCH16 solve_zlean_batch<16, 16>
 #endif
