@@ -1733,14 +1733,14 @@ kernel_config_reads_images(const AcKernelConfig& c)
 bool
 default_reads_images(int nx, int ny, int nz)
 {
-    // FP64: the WRAP build saves three 0.3 ms fills per 512^3 step for a solve about 1 % slower
-    // (29.8 -> 29.2 ms per step). FP32: its WRAP build is slower per launch at 512^3 (4.37 against
-    // 4.08 ms; its box-edge tiles fetch extra halo lines), more than the fills it saves (12.69 ->
-    // 13.20 ms per step), but faster per step at 256^3 (1.87 -> 1.74 ms) and 128^3 (0.352 ->
-    // 0.319 ms): images below 384^3 cells.
+    // The WRAP builds save the three fills of a step, but their launches slow down with the box
+    // (the far-side halo and plane reads: at 1024^3 FP32 a launch takes 51 ms against 36 ms
+    // with fills). Per step, fill + launch against images (ms, interleaved on one box):
+    //   FP64  384^3 13.15 / 13.02   512^3 29.37 / 29.53   768^3 100.0 / 105.1   1024^3 230 / 240
+    //   FP32  256^3 1.67 / 1.58     384^3 6.31 / 6.36     448^3 10.24 / 10.18   512^3 12.78 / 13.44
+    // so images below 512^3 cells in FP64 and below 384^3 in FP32.
 #if AC_DOUBLE_PRECISION == 1
-    (void)nx, (void)ny, (void)nz;
-    return true;
+    return (double)nx * ny * nz < 512.0 * 512.0 * 512.0;
 #else
     return (double)nx * ny * nz < 384.0 * 384.0 * 384.0;
 #endif

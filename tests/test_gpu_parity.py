@@ -485,8 +485,8 @@ def test_step_reads_periodic_images(lib, zd):
 def test_periodic_image_switch(lib):
     """acDeviceSetReadPeriodicImages is refused (AC_FAILURE) for a configuration without an
     image-reading build, and compensated integration turns image reads off. By default a small box
-    reads images and a 512^3 FP32 box does not (the measured crossover lies between 256^3 and
-    512^3 in FP32; FP64 reads them at every size)."""
+    reads images and a 512^3 box does not (the measured crossovers: between 320^3 and 384^3 cells
+    in FP32, between 384^3 and 512^3 in FP64)."""
     info = lib.config(n=24)
     with make_device(lib, info) as dev:
         assert lib.dll.acDeviceSubstepReadsPeriodicImages(dev.handle) == 1
@@ -503,7 +503,7 @@ def test_periodic_image_switch(lib):
         assert lib.dll.acDeviceSubstepReadsPeriodicImages(dev.handle) == 0
     big = lib.config(n=512)
     with make_device(lib, big) as dev:
-        assert lib.dll.acDeviceSubstepReadsPeriodicImages(dev.handle) == (1 if lib.double else 0)
+        assert lib.dll.acDeviceSubstepReadsPeriodicImages(dev.handle) == 0
 
 
 def test_graph_step_matches_eager(lib):
