@@ -708,7 +708,11 @@ def tuning_sweep(double: bool, n: int):
     try:
         device_init_mhd(dev)
         for c in lib.kernel_configs():
-            depths = [0] if c.variant == 0 else ([8, 16, 32, 64, 0] if c.variant == 1 else [14, 28, 56, 0])
+            # the library tuner's candidates: powers of two for variant 1, whole 8-plane rounds for
+            # variant 13, the box depth cut in 2, 3, 4 or 8 for the other lean marches
+            depths = [0] if c.variant == 0 else ([8, 16, 32, 64, 128, 0] if c.variant == 1 else
+                                                 [16, 32, 64, 128, 0] if c.variant == 13 else
+                                                 [(n + 1) // 2, (n + 2) // 3, (n + 3) // 4, (n + 7) // 8, 0])
             for zd in depths:
                 if zd >= n:
                     continue
