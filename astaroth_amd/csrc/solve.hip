@@ -43,6 +43,7 @@
 #include "physics.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 
 namespace ac {
@@ -1765,18 +1766,26 @@ kernel_config_at(int index)
     return c;
 }
 
-// Compute units of the current GPU (read once; 256 if the query fails).
+// Compute units of the current GPU (queried once per device; 256 if the query fails).
 static int
 current_cu_count()
 {
-    static const int cus = [] {
-        int dev = 0, n = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
-            n = 256;
-        return n;
-    }();
-    return cus;
+    constexpr int MAXDEV = 64;
+    static std::atomic<int> known[MAXDEV]; // 0: not queried yet
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0)
+        return 256;
+    if (dev < MAXDEV) {
+        const int c = known[dev].load(std::memory_order_relaxed);
+        if (c > 0)
+            return c;
+    }
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+        n = 256;
+    if (dev < MAXDEV)
+        known[dev].store(n, std::memory_order_relaxed);
+    return n;
 }
 
 int

@@ -27,6 +27,15 @@ class Hip:
         d.hipMemset.argtypes = [vp, ctypes.c_int, ctypes.c_size_t]
         d.hipDeviceSynchronize.argtypes = []
 
+    # hipDeviceAttributeMultiprocessorCount in hip_runtime_api.h's hipDeviceAttribute_t
+    _ATTR_MULTIPROCESSOR_COUNT = 63
+
+    def cu_count(self, device: int = 0) -> int:
+        n = ctypes.c_int()
+        if self.dll.hipDeviceGetAttribute(ctypes.byref(n), self._ATTR_MULTIPROCESSOR_COUNT, device) != 0:
+            raise RuntimeError("hipDeviceGetAttribute failed")
+        return n.value
+
     # ---- events ----
     def event(self):
         ev = ctypes.c_void_p()

@@ -580,6 +580,30 @@ def test_uniform_loaders_reject_invalid_values(lib):
                                                  info, abi.int3(0, 0, 0)) == abi.AC_FAILURE
 
 
+@pytest.mark.parametrize("dims", [(256, 256, 256), (128, 128, 128), (96, 64, 200), (512, 512, 512)])
+def test_default_z_chunks_are_equal_and_fill_the_gpu(lib, dims):
+    """The default configuration cuts the tile columns into equal z-chunks (lengths differing
+    by less than the number of chunks) and uses whole columns once the columns alone fill the
+    resident block slots (FP32: two 64 KiB blocks per CU, FP64: one); with fewer columns than
+    slots its chunks occupy at least three quarters of them (one round of blocks, slightly short
+    of the slots, beats a second round: (96, 64, 200) in FP32 is 20 chunks of 10 planes, 480
+    blocks on 512 slots). Whole 512^3 columns: 1024 blocks."""
+    from astaroth_amd.hiprt import Hip
+    slots = (1 if lib.double else 2) * Hip().cu_count(0)
+    nx, ny, nz = dims
+    with make_device(lib, lib.config(n=nx, ny=ny, nz=nz)) as dev:
+        c = dev.kernel_config()
+    assert (c.variant, c.tile_x, c.tile_y) == ((8, 16, 16) if lib.double else (19, 16, 16))
+    cols = ((nx + 15) // 16) * ((ny + 15) // 16)
+    if c.z_depth == 0:
+        assert cols * 4 >= slots or nz < 16, (dims, cols, slots)
+        return
+    k = -(-nz // c.z_depth)
+    chunks = [min(c.z_depth, nz - i * c.z_depth) for i in range(k)]
+    assert max(chunks) - min(chunks) < k, chunks
+    assert cols * k >= 0.75 * slots, (cols, k, slots)
+
+
 def test_autotune_selects_a_valid_config():
     lib = Library(False)
     n = 48
