@@ -26,8 +26,8 @@ using namespace ac;
 namespace {
 
 constexpr int NDIAG     = 7;  // per-box diagnostics, see acEnsembleDiagnostics
-constexpr int DIAG_BLKS = 64; // blocks per box of the diagnostics launch
-constexpr int DIAG_UNROLL = 4; // cells per lane whose loads are in flight together
+constexpr int DIAG_CELLS  = 16384; // interior cells per block of the diagnostics launch
+constexpr int DIAG_UNROLL = 4;     // cells per lane whose loads are in flight together
 constexpr int FILL_BLOCK  = 256; // threads per block of the batched fill
 
 // The field pointers of a box come from a device array, so the compiler cannot tell their
@@ -87,6 +87,7 @@ struct DiagBox {
     const AcReal* f[NF];
     int nx, ny, nz, mx;
     long mxy;
+    int first_blk; // the box's first block of the diagnostics launch (ceil(cells / DIAG_CELLS) each)
 };
 
 __device__ __forceinline__ double
@@ -108,20 +109,30 @@ is_max(int q)
     return q == 0 || q == 2 || q == 4;
 }
 
-// The interior of box blockIdx.y walked as one flat list of cells (x fastest, rows back to
-// back), so short rows leave no lane idle; each lane has the loads of DIAG_UNROLL cells in flight
-// before it accumulates them.
+// Every block takes DIAG_CELLS cells of one box's interior, walked as one flat list (x fastest,
+// rows back to back), so that short rows leave no lane idle and every block does the same work
+// whatever the box's size; the block finds its box by a search keyed by its index alone
+// (uniform). Each lane has the loads of DIAG_UNROLL cells in flight before it accumulates them.
 __global__ void __launch_bounds__(256)
-diagnostics_batch(const DiagBox* __restrict__ boxes, double* __restrict__ partials)
+diagnostics_batch(const DiagBox* __restrict__ boxes, const int nboxes, double* __restrict__ partials)
 {
     __shared__ double lds[4][NDIAG];
-    const DiagBox& b = boxes[blockIdx.y];
+    int lo = 0, hi = nboxes - 1; // last box whose first block is <= blockIdx.x
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (boxes[mid].first_blk <= (int)blockIdx.x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const DiagBox& b = boxes[lo];
     double v[NDIAG]  = {0.0, 0.0, 0.0, 0.0, -INFINITY, INFINITY, 0.0};
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned nx = (unsigned)b.nx, nxy = (unsigned)b.nx * (unsigned)b.ny;
-    const unsigned ncell  = nxy * (unsigned)b.nz;
-    const unsigned stride = gridDim.x * 256u;
-    for (unsigned c0 = blockIdx.x * 256u + threadIdx.x; c0 < ncell; c0 += DIAG_UNROLL * stride) {
+    const unsigned cbeg   = (blockIdx.x - (unsigned)b.first_blk) * (unsigned)DIAG_CELLS;
+    const unsigned ncell  = min(nxy * (unsigned)b.nz, cbeg + (unsigned)DIAG_CELLS);
+    const unsigned stride = 256u;
+    for (unsigned c0 = cbeg + threadIdx.x; c0 < ncell; c0 += DIAG_UNROLL * stride) {
         AcReal f[DIAG_UNROLL][7];
         bool ok[DIAG_UNROLL];
 #pragma unroll
@@ -172,21 +183,23 @@ diagnostics_batch(const DiagBox* __restrict__ boxes, double* __restrict__ partia
         double x    = lds[0][q];
         for (int w = 1; w < 4; ++w)
             x = q == 5 ? dmin(x, lds[w][q]) : is_max(q) ? dmax(x, lds[w][q]) : x + lds[w][q];
-        partials[((long)blockIdx.y * gridDim.x + blockIdx.x) * NDIAG + q] = x;
+        partials[(long)blockIdx.x * NDIAG + q] = x;
     }
 }
 
+// Per box and quantity, the box's block partials folded in block order.
 __global__ void
-diagnostics_finalize(const double* __restrict__ partials, const int nboxes, const int nblk,
-                     double* __restrict__ results)
+diagnostics_finalize(const double* __restrict__ partials, const DiagBox* __restrict__ boxes,
+                     const int nboxes, const int nblk, double* __restrict__ results)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nboxes * NDIAG)
         return;
     const int box = t / NDIAG, q = t % NDIAG;
-    double x = partials[((long)box * nblk) * NDIAG + q];
-    for (int i = 1; i < nblk; ++i) {
-        const double y = partials[((long)box * nblk + i) * NDIAG + q];
+    const int b0 = boxes[box].first_blk, b1 = box + 1 < nboxes ? boxes[box + 1].first_blk : nblk;
+    double x = partials[(long)b0 * NDIAG + q];
+    for (int i = b0 + 1; i < b1; ++i) {
+        const double y = partials[(long)i * NDIAG + q];
         x              = q == 5 ? dmin(x, y) : is_max(q) ? dmax(x, y) : x + y;
     }
     results[t] = x;
@@ -216,6 +229,7 @@ struct ensemble_s {
     FillBox* d_fill[2];
     long fill_total;
     DiagBox* d_diag[2];
+    int diag_blocks; // blocks of the diagnostics launch, DIAG_CELLS interior cells each
     double* d_partials;
     double* d_results;
     double* h_results; // pinned
@@ -368,14 +382,14 @@ enqueue_fill(Ensemble e, int par)
 hipError_t
 enqueue_diagnostics(Ensemble e, int par)
 {
-    diagnostics_batch<<<dim3(DIAG_BLKS, e->nboxes), dim3(256), 0, e->stream>>>(e->d_diag[par],
-                                                                               e->d_partials);
+    diagnostics_batch<<<dim3((unsigned)e->diag_blocks), dim3(256), 0, e->stream>>>(
+        e->d_diag[par], e->nboxes, e->d_partials);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess)
         return err;
     const int n = e->nboxes * NDIAG;
     diagnostics_finalize<<<dim3((n + 255) / 256), dim3(256), 0, e->stream>>>(
-        e->d_partials, e->nboxes, DIAG_BLKS, e->d_results);
+        e->d_partials, e->d_diag[par], e->nboxes, e->diag_blocks, e->d_results);
     return hipGetLastError();
 }
 
@@ -520,10 +534,12 @@ acEnsembleCreate(const int device_id, const int nboxes, const AcMeshInfo* config
     // fill and diagnostics descriptors per parity
     std::vector<FillBox> fb(nboxes);
     std::vector<DiagBox> db(nboxes);
+    int diag_blocks = 0;
     for (int p = 0; p < 2; ++p) {
         AC_HIP_CHECK(hipMalloc((void**)&e->d_fill[p], nboxes * sizeof(FillBox)));
         AC_HIP_CHECK(hipMalloc((void**)&e->d_diag[p], nboxes * sizeof(DiagBox)));
-        long first = 0;
+        long first  = 0;
+        diag_blocks = 0; // the same for both parities
         for (int b = 0; b < nboxes; ++b) {
             const int3 n = dims_of(e->info[b]);
             for (int q = 0; q < NF; ++q) {
@@ -546,6 +562,8 @@ acEnsembleCreate(const int device_id, const int nboxes, const AcMeshInfo* config
             db[b].nz  = n.z;
             db[b].mx  = n.x + 6;
             db[b].mxy = (long)(n.x + 6) * (n.y + 6);
+            db[b].first_blk = diag_blocks;
+            diag_blocks += (int)(((long)n.x * n.y * n.z + DIAG_CELLS - 1) / DIAG_CELLS);
         }
         e->fill_total = first;
         AC_HIP_CHECK(hipMemcpy(e->d_fill[p], fb.data(), nboxes * sizeof(FillBox),
@@ -553,7 +571,8 @@ acEnsembleCreate(const int device_id, const int nboxes, const AcMeshInfo* config
         AC_HIP_CHECK(hipMemcpy(e->d_diag[p], db.data(), nboxes * sizeof(DiagBox),
                                hipMemcpyHostToDevice));
     }
-    AC_HIP_CHECK(hipMalloc((void**)&e->d_partials, (size_t)nboxes * DIAG_BLKS * NDIAG * sizeof(double)));
+    e->diag_blocks = diag_blocks;
+    AC_HIP_CHECK(hipMalloc((void**)&e->d_partials, (size_t)diag_blocks * NDIAG * sizeof(double)));
     AC_HIP_CHECK(hipMalloc((void**)&e->d_results, (size_t)nboxes * NDIAG * sizeof(double)));
     AC_HIP_CHECK(hipMemset(e->d_results, 0, (size_t)nboxes * NDIAG * sizeof(double)));
     AC_HIP_CHECK(hipHostMalloc((void**)&e->h_results, (size_t)nboxes * NDIAG * sizeof(double)));
